@@ -1,0 +1,287 @@
+"""Benchmark: qLogNEI value+grad evals/s (n=2048, d=12, b=512, q=4, S=512) on one
+MI355X, mixed precision vs the plain FP64 GPU path, plus MLL-grad/s (n=4096, d=16).
+
+One "step" is one batched qLogNEI value + d/dX over all 512 restarts (one
+L-BFGS-B step with every restart active), inputs resident in HBM.  ``value`` is
+steps/s of the mixed path.  Baseline legs (FP64 oracle on PyTorch-ROCm, FP64
+oracle on the host CPU) use the test oracle as the comparator only.
+
+Prints ONE JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "qLogNEI value+grad evals/s (n=2048,d=12), mixed vs FP64 GPU; MLL-grad/s"
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _timed(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def build_headline(precision: str, n=2048, d=12, b=512, q=4, S=512):
+    import botorch_amd as B
+    from botorch_amd import workloads
+    from oracle import gp_oracle as O
+
+    w = workloads.gp_headline(n=n, d=d, b=b, q=q, S=S)
+    dev = "cuda"
+    base = B.MaternKernel(ard_num_dims=d)
+    base.lengthscale = w["lengthscale"]
+    covar = B.ScaleKernel(base)
+    covar.outputscale = 1.0
+    lik = B.GaussianLikelihood()
+    lik.noise = w["noise"]
+    gp = B.SingleTaskGP(w["X"].to(dev), w["Y"].to(dev), likelihood=lik, covar_module=covar)
+    orc = O.OracleGP(w["X"].to(dev), w["Y"].to(dev), O.MATERN52, w["lengthscale"].to(dev), 1.0,
+                     w["noise"])
+    Xb, _ = O.prune_inferior_points(orc, w["X"].to(dev), num_samples=2048, seed=123)
+    sampler = B.SobolQMCNormalSampler(torch.Size([S]), seed=0)
+    acq = B.qLogNoisyExpectedImprovement(gp, Xb, sampler=sampler, prune_baseline=False,
+                                         precision=precision)
+    ref = O.OracleQLogNEI(orc, Xb, num_samples=S, seed=0)
+    return acq, ref, w, orc, Xb
+
+
+def headline_flops(n, d, b, q, n_b, S):
+    """Algorithmic FLOPs of one value+grad step (triangular contraction with
+    L^-1 forward and adjoint, kernel rows, Sigma_qb, samples, reductions)."""
+    m = n_b + q
+    return (2 * b * q * n * n + 3 * b * q * n * (2 * d + 8) + 6 * b * q * n * n_b
+            + 6 * b * q * m * S + 40 * b * q * S)
+
+
+def gemm_fwd_roofline(acq, X, steps, precision):
+    """Times the forward contraction GEMM of a step on its own (same arguments
+    the step issues) with events on the launch stream."""
+    from botorch_amd import _lib
+    L = _lib.lib()
+    cache = acq._keep["cache"]
+    n_pad = cache["n_pad"]
+    b, q, d = X.shape
+    R_pad = _lib.round_up(b * q, 128)
+    ldv = n_pad + acq._state.ext_pad
+    dt = torch.float32 if precision == "mixed" else torch.float64
+    kx = torch.randn(R_pad, n_pad, dtype=dt, device="cuda")
+    vt = torch.empty(R_pad, ldv, dtype=dt, device="cuda")
+    flags = _lib.GEMM_B_LOWER | (_lib.GEMM_F32 if precision == "mixed" else 0)
+    linv_ext = acq._keep["linv_ext"]
+
+    def launch():
+        _lib.check(L.bta_gemm(flags, R_pad, ldv, n_pad, 1.0, _lib.ptr(kx), n_pad,
+                              _lib.ptr(linv_ext), n_pad, 0.0, None, 0, _lib.ptr(vt), ldv, 1,
+                              0, 0, 0, 0, _lib.stream()))
+
+    for _ in range(3):
+        launch()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record()
+    for _ in range(steps):
+        launch()
+    s1.record()
+    torch.cuda.synchronize()
+    avg_ms = s0.elapsed_time(s1) / steps
+    # algorithmic flops: K_x L^-T over the lower triangle + the 1 + n_b dense rows
+    n = cache["model"].n
+    n_b = acq._n_b
+    flops = 2.0 * b * q * (n * (n + 1) / 2 + n * (1 + n_b))
+    return avg_ms, flops
+
+
+def cpu_baseline_headline(w, Xb, budget_s=15.0):
+    """FP64 oracle (PyTorch CPU, reference op sequence) on a bounded sample of
+    the headline workload: value+grad over a slice of the restarts."""
+    from oracle import gp_oracle as O
+    cores = os.cpu_count() or 1
+    torch.set_num_threads(cores)
+    orc = O.OracleGP(w["X"], w["Y"], O.MATERN52, w["lengthscale"], 1.0, w["noise"])
+    ref = O.OracleQLogNEI(orc, Xb.cpu(), num_samples=w["S"], seed=0)
+    bs = 16
+    Xs = w["Xq"][:bs]
+    ref.value_and_grad(Xs)  # warm-up
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < budget_s:
+        ref.value_and_grad(Xs)
+        reps += 1
+    dt = (time.perf_counter() - t0) / reps
+    b = w["Xq"].shape[0]
+    return {"value": 1.0 / (dt * b / bs), "unit": "value+grad evals/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{bs} of {b} restarts per eval ({reps} evals, {reps * dt:.1f}s), "
+                      f"time scaled x{b // bs} to a full {b}-restart step"}
+
+
+def build_mll(n=4096, d=16):
+    from botorch_amd import workloads
+    w = workloads.gp_mll(n=n, d=d)
+    X = w["X"].to("cuda")
+    f = workloads.prior_draw(X, w["z"].to("cuda"), w["lengthscale"])
+    y = f + w["noise_sd"] * w["eps"].to("cuda")
+    y = (y - y.mean()) / y.std()
+    return X.contiguous(), y.contiguous()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--no-mll", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
+              "(restarts shard across GPUs by design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    from botorch_amd import _lib
+    from oracle import gp_oracle as O
+
+    n, d, b, q, S = 2048, 12, 512, 4, 512
+    out = {}
+    # ---------------- mixed-precision HIP path (the measured value) --------------
+    log("building headline model / acquisition (mixed)")
+    acq_m, ref, w, orc, Xb = build_headline("mixed", n, d, b, q, S)
+    X = w["Xq"].to("cuda")
+    n_b = Xb.shape[0]
+
+    def step_mixed():
+        v, g, st = acq_m._launch(X, True)
+        return v, g
+
+    log("timing mixed steps")
+    t_mixed = _timed(step_mixed, args.steps, args.warmup)
+    v_m, g_m, st_m = acq_m._launch(X, True)
+    acq_m.check_status(st_m)
+    log(f"mixed: {t_mixed * 1e3:.3f} ms/step")
+    # ---------------- FP64 HIP path ------------------------------------------------
+    acq_d, *_ = build_headline("fp64", n, d, b, q, S)
+    t_fp64 = _timed(lambda: acq_d._launch(X, True), max(10, args.steps // 2), args.warmup)
+    v_d, g_d, st_d = acq_d._launch(X, True)
+    acq_d.check_status(st_d)
+    log(f"fp64: {t_fp64 * 1e3:.3f} ms/step")
+    # ---------------- accuracy vs the FP64 oracle (same inputs, on device) ---------
+    v_ref, g_ref = ref.value_and_grad(X)
+
+    def rel(a, r):
+        return float(((a - r).abs().max() / r.abs().max()).item())
+
+    acc = {"mixed_value": rel(v_m, v_ref), "mixed_grad": rel(g_m, g_ref),
+           "fp64_value": rel(v_d, v_ref), "fp64_grad": rel(g_d, g_ref)}
+    log(f"accuracy {acc}")
+    # ---------------- plain FP64 GPU path (oracle through PyTorch-ROCm) -----------
+    t_torch = None
+    if not args.no_baselines:
+        t_torch = _timed(lambda: ref.value_and_grad(X), 5, 2)
+        log(f"torch fp64 GPU: {t_torch * 1e3:.3f} ms/step")
+    # ---------------- roofline of the dominant kernel ------------------------------
+    gemm_ms, gemm_flops = gemm_fwd_roofline(acq_m, X, 50, "mixed")
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS["f32"],
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["f32"], 4),
+            "traffic": None, "kernel": "gemm_kernel<float,...> forward K_x [L^-1;alpha;A]^T",
+            "avg_launch_ms": round(gemm_ms, 5), "flops_per_launch": gemm_flops}
+    step_flops = headline_flops(n, d, b, q, n_b, S)
+    # ---------------- MLL gradient ----------------------------------------------
+    mll = None
+    if not args.no_mll:
+        import ctypes
+        log("MLL n=4096 d=16")
+        Xm, ym = build_mll()
+        nm, dm = Xm.shape
+        L = _lib.lib()
+        ws = _lib.workspace(L.bta_mll_workspace_size(nm, dm), "cuda")
+        ls = [0.4] * dm
+        lsh = (ctypes.c_double * dm)(*ls)
+        outb = (ctypes.c_double * (4 + dm))()
+
+        def mll_step():
+            _lib.check(L.bta_mll_value_grad(0, _lib.ptr(Xm), _lib.ptr(ym), nm, dm, lsh, 1.0,
+                                            0.05**2, 0.0, 6, outb, _lib.ptr(ws), ws.numel(),
+                                            _lib.stream()))
+
+        t_mll = _timed(mll_step, 10, 2)
+        g_hip = torch.tensor(list(outb), dtype=torch.float64)
+        log(f"MLL hip: {t_mll * 1e3:.2f} ms")
+        lsr = torch.full((dm,), 0.4, dtype=torch.float64, device="cuda", requires_grad=True)
+        osr = torch.tensor(1.0, dtype=torch.float64, device="cuda", requires_grad=True)
+        nr = torch.tensor(0.05**2, dtype=torch.float64, device="cuda", requires_grad=True)
+        cr = torch.tensor(0.0, dtype=torch.float64, device="cuda", requires_grad=True)
+
+        def torch_mll():
+            for p in (lsr, osr, nr, cr):
+                p.grad = None
+            ll = O.exact_mll(O.MATERN52, Xm, ym, lsr, osr, nr, cr) * nm
+            ll.backward()
+            return ll
+
+        t_mll_torch = None
+        ll = torch_mll()
+        g_ref_m = torch.cat([torch.stack([ll.detach(), osr.grad, nr.grad, cr.grad]),
+                             lsr.grad]).cpu()
+        if not args.no_baselines:
+            t_mll_torch = _timed(torch_mll, 3, 1)
+            log(f"MLL torch fp64: {t_mll_torch * 1e3:.2f} ms")
+        mll = {"workload": "exact MLL value+grad, n=4096, d=16, Matern-5/2 ARD",
+               "mll_grad_per_s": 1.0 / t_mll, "ms": t_mll * 1e3,
+               "torch_fp64_gpu_per_s": (1.0 / t_mll_torch) if t_mll_torch else None,
+               "speedup_vs_fp64_gpu": (t_mll_torch / t_mll) if t_mll_torch else None,
+               "max_rel_err_grad": float(((g_hip[1:] - g_ref_m[1:]).abs().max()
+                                          / g_ref_m[1:].abs().max()).item()),
+               "rel_err_loglik": float(abs(g_hip[0] - g_ref_m[0]) / abs(g_ref_m[0])),
+               "flops": nm**3 / 3 + 2 * nm**3 / 3 + nm * nm * (5 * dm + 18) / 2}
+    # ---------------- CPU baseline --------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline_headline(w, Xb)
+    res = {
+        "metric": METRIC, "value": 1.0 / t_mixed, "unit": "value+grad evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (MFMA contraction) + f64 (factors, MC)",
+        "data": "synthetic",
+        "config": {"workload": "qLogNEI value+grad, SingleTaskGP Matern-5/2 ARD n=2048 d=12, "
+                               "b=512 restarts x q=4, S=512 Sobol samples",
+                   "n": n, "d": d, "restarts": b, "q": q, "mc_samples": S, "n_baseline": n_b,
+                   "precision": "mixed"},
+        "restart_evals_per_s": b / t_mixed,
+        "fp64_hip_steps_per_s": 1.0 / t_fp64,
+        "torch_fp64_gpu_steps_per_s": (1.0 / t_torch) if t_torch else None,
+        "speedup_mixed_vs_fp64_gpu": (t_torch / t_mixed) if t_torch else None,
+        "speedup_fp64hip_vs_fp64_gpu": (t_torch / t_fp64) if t_torch else None,
+        "step_tflops_mixed": step_flops / t_mixed / 1e12,
+        "accuracy_max_rel_err_vs_fp64_oracle": acc,
+        "roofline": roof, "cpu_baseline": cpu, "mll": mll,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,251 @@
+"""qLogNoisyExpectedImprovement on libbotorch_amd.
+
+Mirrors acquisition/logei.py:246-585 (qLogNoisyExpectedImprovement with
+cache_root=True, incremental, fat=True), acquisition/monte_carlo.py:146-348
+(SampleReducingMCAcquisitionFunction.forward: fat-max over q, logmeanexp over
+samples), acquisition/cached_cholesky.py:63-192 (cached baseline root and the
+split-seed base samples) and acquisition/utils.py:347-437
+(prune_inferior_points).
+
+``forward(X)`` runs the whole value (and, when X requires grad, the d/dX
+adjoint) in one library call; the autograd Function hands the precomputed
+gradient back in ``backward``.  ``value_and_grad`` is the optimiser entry.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import warnings
+
+import torch
+from torch import Tensor
+
+from . import _lib
+from .exceptions import NanError, NotPSDError, UnsupportedError
+from .gp import SingleTaskGP
+from .sampling import SobolQMCNormalSampler, draw_sobol_normal_samples
+
+TAU_RELU = 1e-6  # acquisition/logei.py:67
+TAU_MAX = 1e-2  # acquisition/logei.py:68
+
+
+def prune_inferior_points(model: SingleTaskGP, X: Tensor, objective=None,
+                          posterior_transform=None, constraints=None, num_samples: int = 2048,
+                          max_frac: float = 1.0, sampler=None, marginalize_dim=None) -> Tensor:
+    """Keep the points of X that are the argmax of at least one posterior sample
+    (acquisition/utils.py:347-437)."""
+    if objective is not None or posterior_transform is not None or constraints is not None:
+        raise UnsupportedError("objectives, posterior transforms and constraints are not supported")
+    if X.ndim > 2:
+        raise UnsupportedError("Batched inputs `X` are currently unsupported by "
+                               "`prune_inferior_points`")
+    if X.size(-2) == 0:
+        raise ValueError("X must have at least one point.")
+    if max_frac <= 0 or max_frac > 1.0:
+        raise ValueError(f"max_frac must take values in (0, 1], is {max_frac}")
+    max_points = math.ceil(max_frac * X.size(-2))
+    with torch.no_grad():
+        posterior = model.posterior(X)
+        if sampler is None:
+            sampler = SobolQMCNormalSampler(sample_shape=torch.Size([num_samples]))
+        samples = sampler(posterior)
+    obj_vals = samples.squeeze(-1)
+    is_best = torch.argmax(obj_vals, dim=-1)
+    idcs, counts = torch.unique(is_best, return_counts=True)
+    if len(idcs) > max_points:
+        counts, order_idcs = torch.sort(counts, stable=True, descending=True)
+        idcs = order_idcs[:max_points]
+    return X[idcs]
+
+
+class _QLogNEIFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, X: Tensor, acqf: "qLogNoisyExpectedImprovement", need_grad: bool):
+        val, grad = acqf._run(X.detach(), need_grad)
+        ctx.save_for_backward(grad if grad is not None else torch.empty(0))
+        ctx.has_grad = grad is not None
+        return val
+
+    @staticmethod
+    def backward(ctx, g_out: Tensor):
+        (grad,) = ctx.saved_tensors
+        if not ctx.has_grad:
+            raise RuntimeError("qLogNEI forward ran without gradient")
+        return g_out.reshape(-1, 1, 1) * grad, None, None
+
+
+class qLogNoisyExpectedImprovement(torch.nn.Module):
+    """MC-based batch Log Noisy Expected Improvement (acquisition/logei.py:246)."""
+
+    def __init__(self, model: SingleTaskGP, X_baseline: Tensor, sampler=None, objective=None,
+                 posterior_transform=None, X_pending=None, constraints=None, eta=1e-3,
+                 fat: bool = True, prune_baseline: bool = True, cache_root: bool | None = None,
+                 tau_max: float = TAU_MAX, tau_relu: float = TAU_RELU, marginalize_dim=None,
+                 incremental: bool = True, precision: str = "fp64"):
+        super().__init__()
+        if objective is not None or posterior_transform is not None or constraints is not None:
+            raise UnsupportedError("objectives, posterior transforms and constraints are "
+                                   "outside the accelerated path")
+        if X_pending is not None:
+            raise UnsupportedError("X_pending is not supported")
+        if not fat:
+            raise UnsupportedError("only fat=True (fat-max / log-fatplus) is implemented")
+        if cache_root is False:
+            raise UnsupportedError("only the cached-root path is implemented")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        for name, tau in (("tau_max", tau_max), ("tau_relu", tau_relu)):
+            if not tau > 0:
+                raise ValueError(f"{name} is non-positive: {tau=}.")
+        self.model = model
+        self.sampler = sampler or SobolQMCNormalSampler(sample_shape=torch.Size([512]))
+        if len(self.sampler.sample_shape) != 1:
+            raise UnsupportedError("one sample dimension is supported")
+        self.tau_max = float(tau_max)
+        self.tau_relu = float(tau_relu)
+        self.precision = precision
+        self.prune_baseline = prune_baseline
+        self.X_pending = None
+        if prune_baseline:
+            X_baseline = prune_inferior_points(model=model, X=X_baseline)
+        self._init_baseline(X_baseline)
+
+    # acquisition/logei.py:393-459
+    def _init_baseline(self, X_baseline: Tensor) -> None:
+        L = _lib.lib()
+        model = self.model
+        cache = model.prediction_cache()
+        m = cache["model"]
+        dev = cache["xs"].device
+        Xb = X_baseline.to(device=dev, dtype=torch.float64).contiguous()
+        n_b, d = Xb.shape
+        S = int(self.sampler.sample_shape.numel())
+        ext_pad = _lib.round_up(n_b + 1, 128)
+        n_pad = cache["n_pad"]
+        st = _lib.stream()
+        xb_s = torch.empty(n_b, d, dtype=torch.float64, device=dev)
+        _lib.check(L.bta_scale_inputs(_lib.ptr(Xb), n_b, d, _lib.ptr(cache["center"]),
+                                      _lib.ptr(cache["lengthscale"]), None, _lib.ptr(xb_s), n_b,
+                                      st), "scale_inputs")
+        edtype = torch.float32 if self.precision == "mixed" else torch.float64
+        linv_ext = torch.empty(n_pad + ext_pad, n_pad, dtype=edtype, device=dev)
+        Lb = torch.empty(n_b, n_b, dtype=torch.float64, device=dev)
+        best_f = torch.empty(S, dtype=torch.float64, device=dev)
+        seed = self.sampler.seed
+        Zb = draw_sobol_normal_samples(n_b, S, device=dev, seed=seed).contiguous()
+        state = _lib.QLogNEIState()
+        state.precision = _lib.PREC_MIXED if self.precision == "mixed" else _lib.PREC_FP64
+        state.q = 1
+        state.n_b = n_b
+        state.S = S
+        state.ext_pad = ext_pad
+        state.xb_s = _lib.ptr(xb_s)
+        state.linv_ext = _lib.ptr(linv_ext)
+        state.Lb = _lib.ptr(Lb)
+        state.best_f = _lib.ptr(best_f)
+        state.tau_relu = self.tau_relu
+        state.tau_max = self.tau_max
+        state.max_tries = 6
+        state.jitter = 1e-8
+        ws = _lib.workspace(L.bta_qlognei_prepare_workspace_size(ctypes.byref(m), n_b, S), dev)
+        jit = ctypes.c_double(0.0)
+        _lib.check(L.bta_qlognei_prepare(ctypes.byref(m), ctypes.byref(state), _lib.ptr(Zb),
+                                         _lib.ptr(linv_ext), _lib.ptr(Lb), _lib.ptr(best_f),
+                                         ctypes.byref(jit), _lib.ptr(ws), ws.numel(), st),
+                   "qlognei_prepare")
+        self.X_baseline = Xb
+        self._n_b = n_b
+        self._S = S
+        self._state = state
+        self._keep = {"xb_s": xb_s, "linv_ext": linv_ext, "Lb": Lb, "best_f": best_f,
+                      "Zb": Zb, "cache": cache}
+        self._Zt = {}
+        self._ws = None
+        self._baseline_L = Lb
+        self._baseline_best_f = best_f
+
+    def base_samples_t(self, q: int) -> Tensor:
+        """(n_b + q) x S base samples: baseline columns from the baseline sampler,
+        the new q columns from Sobol(n_b + q) with the same seed
+        (sampling/normal.py:68-135)."""
+        if q not in self._Zt:
+            dev = self.X_baseline.device
+            Z = draw_sobol_normal_samples(self._n_b + q, self._S, device=dev,
+                                          seed=self.sampler.seed)
+            Z[:, : self._n_b] = self._keep["Zb"]
+            self._Zt[q] = Z.t().contiguous()
+        return self._Zt[q]
+
+    def _workspace(self, b: int, q: int) -> Tensor:
+        L = _lib.lib()
+        self._state.q = q
+        need = L.bta_qlognei_workspace_size(ctypes.byref(self._keep["cache"]["model"]),
+                                            ctypes.byref(self._state), b)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = _lib.workspace(need, self.X_baseline.device)
+        return self._ws
+
+    def _launch(self, X: Tensor, need_grad: bool):
+        """Enqueue one value(+grad) evaluation; returns device tensors
+        (value b, grad b x q x d or None, status b)."""
+        L = _lib.lib()
+        b, q, d = X.shape
+        if q > 8:
+            raise UnsupportedError("q <= 8 is supported")
+        Xc = X.to(device=self.X_baseline.device, dtype=torch.float64).contiguous()
+        Zt = self.base_samples_t(q)
+        ws = self._workspace(b, q)
+        st = self._state
+        st.q = q
+        st.Zt = _lib.ptr(Zt)
+        value = torch.empty(b, dtype=torch.float64, device=Xc.device)
+        grad = torch.empty(b, q, d, dtype=torch.float64, device=Xc.device) if need_grad else None
+        status = torch.empty(b, dtype=torch.int32, device=Xc.device)
+        _lib.check(L.bta_qlognei_value_grad(ctypes.byref(self._keep["cache"]["model"]),
+                                            ctypes.byref(st), _lib.ptr(Xc), b, _lib.ptr(value),
+                                            _lib.ptr(grad), _lib.ptr(status), _lib.ptr(ws),
+                                            ws.numel(), _lib.stream()), "qlognei_value_grad")
+        return value, grad, status
+
+    @staticmethod
+    def check_status(status: Tensor) -> None:
+        """Raise like the reference would on failed samples (utils/low_rank.py:162-171,
+        psd_safe_cholesky NotPSDError); warn when jitter was added."""
+        s = torch.zeros(3, dtype=torch.bool, device=status.device)
+        for i, f in enumerate((_lib.STATUS_NAN, _lib.STATUS_NOT_PSD, _lib.STATUS_JITTER)):
+            s[i] = ((status & f) != 0).any()
+        nan, notpsd, jit = s.tolist()
+        if nan:
+            raise NanError("Samples contain nans or infs.")
+        if notpsd:
+            raise NotPSDError("Matrix not positive definite after repeatedly adding jitter.")
+        if jit:
+            warnings.warn("A not p.d., added jitter to the diagonal", RuntimeWarning)
+
+    def _run(self, X: Tensor, need_grad: bool):
+        value, grad, status = self._launch(X, need_grad)
+        self.check_status(status)
+        return value, grad
+
+    def value_and_grad(self, X: Tensor):
+        """Batched value and d value[b] / d X[b] for X: b x q x d (one L-BFGS step)."""
+        Xt = X if X.dim() > 2 else X.unsqueeze(0)
+        value, grad = self._run(Xt.detach(), True)
+        return value, grad
+
+    def forward(self, X: Tensor) -> Tensor:
+        """SampleReducingMCAcquisitionFunction.forward with t_batch_mode_transform
+        (utils/transforms.py:239-323): X is batch_shape x q x d."""
+        if X.dim() < 2:
+            raise ValueError(f"{type(self).__name__} requires X to have at least 2 dimensions,"
+                             f" but received X with only {X.dim()} dimensions.")
+        Xt = X if X.dim() > 2 else X.unsqueeze(0)
+        batch_shape = Xt.shape[:-2]
+        q, d = Xt.shape[-2:]
+        Xf = Xt.reshape(-1, q, d)
+        need_grad = torch.is_grad_enabled() and Xf.requires_grad
+        if need_grad:
+            val = _QLogNEIFn.apply(Xf, self, True)
+        else:
+            val, _ = self._run(Xf.detach(), False)
+        return val.reshape(batch_shape).to(X.dtype)

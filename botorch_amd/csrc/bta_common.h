@@ -1,0 +1,74 @@
+// Shared definitions for the botorch_amd HIP library (gfx950 / CDNA4).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/botorch_amd.h"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+namespace bta {
+
+// Thread-local last-error text, returned by bta_last_error().
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+int num_cus();
+
+#define BTA_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t _e = (call);                                             \
+    if (_e != hipSuccess) return ::bta::hip_fail(_e, #call);            \
+  } while (0)
+
+#define BTA_LAUNCH_CHECK(what)                                          \
+  do {                                                                  \
+    hipError_t _e = hipGetLastError();                                  \
+    if (_e != hipSuccess) return ::bta::hip_fail(_e, what);             \
+  } while (0)
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Matern-5/2 / RBF value and the radial factor of its gradient, in scaled
+// coordinates (x / lengthscale).  For r^2 = |xs - ts|^2:
+//   Matern-5/2: k = s (1 + sqrt5 r + 5/3 r^2) exp(-sqrt5 r)
+//               dk/dxs = -s 5/3 (1 + sqrt5 r) exp(-sqrt5 r) (xs - ts) = -h (xs - ts)
+//   RBF:        k = s exp(-r^2/2),  dk/dxs = -s exp(-r^2/2) (xs - ts)
+__device__ __forceinline__ double kern_value(int kind, double r2, double s) {
+  if (kind == BTA_KERNEL_MATERN52) {
+    double r = sqrt(r2);
+    double sr = 2.2360679774997896964 * r;
+    return s * (1.0 + sr + (5.0 / 3.0) * r2) * exp(-sr);
+  }
+  return s * exp(-0.5 * r2);
+}
+
+// returns h with dk/dxs = -h (xs - ts); also the value through *k
+__device__ __forceinline__ double kern_grad_factor(int kind, double r2, double s, double* k) {
+  if (kind == BTA_KERNEL_MATERN52) {
+    double r = sqrt(r2);
+    double sr = 2.2360679774997896964 * r;
+    double e = exp(-sr);
+    if (k) *k = s * (1.0 + sr + (5.0 / 3.0) * r2) * e;
+    return s * (5.0 / 3.0) * (1.0 + sr) * e;
+  }
+  double e = exp(-0.5 * r2);
+  if (k) *k = s * e;
+  return s * e;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+}  // namespace bta

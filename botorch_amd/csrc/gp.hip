@@ -1,0 +1,331 @@
+// SingleTaskGP prediction caches, posterior, and the qLogNEI baseline cache.
+//
+// Cache build (once per model, fast_pred_var form):
+//   K = k(X,X) + noise I  ->  L = chol(K)  ->  L^-1  ->  alpha = L^-T L^-1 (y - c)
+// Posterior over batch x mq points (original outcome scale):
+//   V = k(X*, X) L^-T;  mean = y_mean + y_std (c + k(X*,X) alpha)
+//   covar = y_std^2 (k(X*,X*) - V V^T)
+// qLogNEI baseline cache (acquisition/logei.py:393-459):
+//   A^T = V_b L^-1 (= (K^-1 k(X, X_b))^T), Lb = chol(Sigma_bb), best_f from the
+//   baseline samples, and the extended operator [L^-1; alpha^T; A^T] used by
+//   both GEMMs of every value+grad step.
+#include "bta_common.h"
+#include "gemm.h"
+#include "kernels.h"
+
+#include <algorithm>
+
+namespace bta {
+
+// rows [i*mq_pad + r] of Xs = scaled X[i*mq + r] (zero rows in the padding)
+__global__ void scale_batched_kernel(const double* X, int64_t batch, int mq, int mq_pad, int d,
+                                     const double* center, const double* ls, double* Xs) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tot = batch * mq_pad * d;
+  if (idx >= tot) return;
+  const int j = (int)(idx % d);
+  const int64_t row = idx / d;
+  const int64_t bi = row / mq_pad;
+  const int r = (int)(row % mq_pad);
+  Xs[idx] = r < mq ? (X[(bi * mq + r) * d + j] - center[j]) / ls[j] : 0.0;
+}
+
+// mean[i*mq+r] = y_mean + y_std (c + KA[(i*mq_pad + r)*lda]);
+// covar/var from the padded batch blocks of Sg (already in the outcome scale)
+__global__ void posterior_out_kernel(const double* KA, int64_t lda, const double* Sg,
+                                     int64_t batch, int mq, int mq_pad, double y_mean,
+                                     double y_std, double c, double* mean, double* covar,
+                                     double* var) {
+  const int64_t bi = blockIdx.y;
+  const double* Sb = Sg + bi * mq_pad * mq_pad;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < mq * mq; e += gridDim.x * blockDim.x) {
+    const int i = e / mq, j = e % mq;
+    const double v = Sb[i * mq_pad + j];
+    if (covar) covar[(bi * mq + i) * mq + j] = v;
+    if (i == j) {
+      if (var) var[bi * mq + i] = v;
+      if (mean) mean[bi * mq + i] = y_mean + y_std * (c + KA[(bi * mq_pad + i) * lda]);
+    }
+  }
+}
+
+__global__ void residual_kernel(const double* y, int64_t n, double c, double* r, int64_t n_pad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_pad) r[i] = i < n ? y[i] - c : 0.0;
+}
+
+__global__ void fill_double_kernel(double* p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+static int fill(double* p, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return BTA_OK;
+  hipLaunchKernelGGL(fill_double_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n,
+                     v);
+  BTA_LAUNCH_CHECK("fill_double_kernel");
+  return BTA_OK;
+}
+
+// alpha = Linv^T Linv r, computed as two 64-row MFMA products
+// (row 0 carries the vector, rows 1..63 are zero).
+static int solve_alpha(const double* Linv, int64_t n_pad, const double* r, double* R1,
+                       double* T1, double* alpha_row, hipStream_t s) {
+  int rc = fill(R1, 64 * n_pad, 0.0, s);
+  if (rc) return rc;
+  BTA_HIP(hipMemcpyAsync(R1, r, n_pad * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // T1[0][i] = sum_k r[k] Linv[i][k]
+  rc = gemm(BTA_GEMM_B_LOWER, 64, n_pad, n_pad, 1.0, R1, n_pad, Linv, n_pad, 0.0, nullptr, 0, T1,
+            n_pad, s);
+  if (rc) return rc;
+  // alpha[j] = sum_i T1[0][i] Linv[i][j]
+  return gemm(BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, 64, n_pad, n_pad, 1.0, T1, n_pad, Linv, n_pad,
+              0.0, nullptr, 0, alpha_row, n_pad, s);
+}
+
+size_t gp_cache_ws(int64_t n_pad) {
+  WsPlan p;
+  p.take(std::max(potrf_ws_bytes(n_pad), trtri_ws_bytes(n_pad)));
+  p.take((size_t)n_pad * 8);
+  p.take((size_t)64 * n_pad * 8 * 3);
+  return p.off;
+}
+
+int gp_cache_build(const bta_gp_model* m, const double* y, double* L, double* Linv,
+                   double* alpha, int max_tries, double* jitter_used, void* ws, size_t ws_bytes,
+                   hipStream_t s) {
+  if (!m || !y || !L || !Linv || !alpha || m->n < 1 || m->n_pad < m->n || m->n_pad % 128) {
+    set_error("gp_cache_build: invalid model (n=%ld n_pad=%ld)", (long)m->n, (long)m->n_pad);
+    return BTA_ERR_INVALID;
+  }
+  if (!ws || ws_bytes < gp_cache_ws(m->n_pad)) {
+    set_error("gp_cache_build: workspace too small");
+    return BTA_ERR_WORKSPACE;
+  }
+  const int64_t np = m->n_pad;
+  WsPlan p;
+  char* base = (char*)ws;
+  const size_t fact_bytes = std::max(potrf_ws_bytes(np), trtri_ws_bytes(np));
+  void* wf = base + p.take(fact_bytes);
+  double* r = (double*)(base + p.take((size_t)np * 8));
+  double* R1 = (double*)(base + p.take((size_t)64 * np * 8 * 3));
+  double* T1 = R1 + 64 * np;
+  double* A1 = T1 + 64 * np;
+  int rc = covar(m->kind, m->train_xs, m->n, m->train_xs, m->n, m->d, m->outputscale, m->noise,
+                 false, L, np, np, np, s, /*pad_identity=*/1);
+  if (rc) return rc;
+  rc = potrf(L, np, np, Linv, np, max_tries, 1e-8, jitter_used, wf, fact_bytes, s);
+  if (rc) return rc;
+  rc = trtri_lower(L, np, Linv, np, np, wf, fact_bytes, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, y,
+                     m->n, m->mean_constant, r, np);
+  BTA_LAUNCH_CHECK("residual_kernel");
+  rc = solve_alpha(Linv, np, r, R1, T1, A1, s);
+  if (rc) return rc;
+  BTA_HIP(hipMemcpyAsync(alpha, A1, np * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return BTA_OK;
+}
+
+struct PostWs {
+  int mq_pad;
+  int64_t R_pad;
+  size_t xs, kx, vt, ka, kbb, sg, arow, total;
+  PostWs(const bta_gp_model* m, int64_t batch, int mq) {
+    mq_pad = (int)round_up(mq, 64);
+    R_pad = batch * mq_pad;
+    WsPlan p;
+    xs = p.take((size_t)R_pad * m->d * 8);
+    kx = p.take((size_t)R_pad * m->n_pad * 8);
+    vt = p.take((size_t)R_pad * m->n_pad * 8);
+    ka = p.take((size_t)R_pad * 64 * 8);
+    kbb = p.take((size_t)batch * mq_pad * mq_pad * 8);
+    sg = p.take((size_t)batch * mq_pad * mq_pad * 8);
+    arow = p.take((size_t)64 * m->n_pad * 8);
+    total = p.off;
+  }
+};
+
+int gp_posterior(const bta_gp_model* m, const double* X, int64_t batch, int mq, double* mean,
+                 double* covar_out, double* var, void* ws, size_t ws_bytes, hipStream_t s) {
+  if (!m || !X || batch < 1 || mq < 1 || !m->linv || !m->alpha || batch > 65535) {
+    set_error("gp_posterior: invalid arguments");
+    return BTA_ERR_INVALID;
+  }
+  const PostWs w(m, batch, mq);
+  if (!ws || ws_bytes < w.total) {
+    set_error("gp_posterior: workspace too small");
+    return BTA_ERR_WORKSPACE;
+  }
+  char* base = (char*)ws;
+  double* xs = (double*)(base + w.xs);
+  double* kx = (double*)(base + w.kx);
+  double* vt = (double*)(base + w.vt);
+  double* ka = (double*)(base + w.ka);
+  double* kbb = (double*)(base + w.kbb);
+  double* sg = (double*)(base + w.sg);
+  double* arow = (double*)(base + w.arow);
+  const int64_t np = m->n_pad;
+  const int64_t tot = w.R_pad * m->d;
+  hipLaunchKernelGGL(scale_batched_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                     X, batch, mq, w.mq_pad, m->d, m->center, m->lengthscale, xs);
+  BTA_LAUNCH_CHECK("scale_batched_kernel");
+  int rc = covar(m->kind, xs, w.R_pad, m->train_xs, m->n, m->d, m->outputscale, 0.0, false, kx,
+                 np, w.R_pad, np, s);
+  if (rc) return rc;
+  rc = gemm(BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, kx, np, m->linv, np, 0.0, nullptr, 0, vt, np,
+            s);
+  if (rc) return rc;
+  rc = cast_pad<double>(m->alpha, np, 1, np, arow, np, 64, np, s);
+  if (rc) return rc;
+  rc = gemm(0, w.R_pad, 64, np, 1.0, kx, np, arow, np, 0.0, nullptr, 0, ka, 64, s);
+  if (rc) return rc;
+  const int64_t mp = w.mq_pad;
+  rc = covar(m->kind, xs, mp, xs, mp, m->d, m->outputscale, 0.0, false, kbb, mp, mp, mp, s, 0,
+             (int)batch, mp * m->d, mp * m->d, mp * mp);
+  if (rc) return rc;
+  const double v2 = m->y_std * m->y_std;
+  rc = gemm(0, mp, mp, np, -v2, vt, np, vt, np, v2, kbb, mp, sg, mp, s, (int)batch, mp * np,
+            mp * np, mp * mp, mp * mp);
+  if (rc) return rc;
+  dim3 g((unsigned)std::min<int64_t>(64, ((int64_t)mq * mq + 255) / 256), (unsigned)batch);
+  hipLaunchKernelGGL(posterior_out_kernel, g, dim3(256), 0, s, ka, (int64_t)64, sg, batch, mq,
+                     w.mq_pad, m->y_mean, m->y_std, m->mean_constant, mean, covar_out, var);
+  BTA_LAUNCH_CHECK("posterior_out_kernel");
+  return BTA_OK;
+}
+
+struct PrepWs {
+  int64_t nbp;
+  size_t kb, vb, at, kbb, sg, lbp, arow, ka, mu, fact, total;
+  PrepWs(const bta_gp_model* m, int n_b, int S) {
+    nbp = round_up(n_b, 64);
+    const int64_t np = m->n_pad;
+    WsPlan p;
+    kb = p.take((size_t)nbp * np * 8);
+    vb = p.take((size_t)nbp * np * 8);
+    at = p.take((size_t)nbp * np * 8);
+    kbb = p.take((size_t)nbp * nbp * 8);
+    sg = p.take((size_t)nbp * nbp * 8);
+    lbp = p.take((size_t)nbp * nbp * 8);
+    arow = p.take((size_t)64 * np * 8);
+    ka = p.take((size_t)nbp * 64 * 8);
+    mu = p.take((size_t)nbp * 8);
+    fact = p.take(potrf_ws_bytes(nbp));
+    total = p.off;
+  }
+};
+
+int qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st, const double* Zb,
+                    void* linv_ext, double* Lb, double* best_f, double* jitter_used, void* ws,
+                    size_t ws_bytes, hipStream_t s) {
+  if (!m || !st || !Zb || !linv_ext || !Lb || !best_f || !m->linv || !m->alpha || !st->xb_s ||
+      st->n_b < 1 || st->S < 1 || st->ext_pad < st->n_b + 1 || st->ext_pad % 128) {
+    set_error("qlognei_prepare: invalid arguments");
+    return BTA_ERR_INVALID;
+  }
+  const PrepWs w(m, st->n_b, st->S);
+  if (!ws || ws_bytes < w.total) {
+    set_error("qlognei_prepare: workspace too small");
+    return BTA_ERR_WORKSPACE;
+  }
+  char* base = (char*)ws;
+  double* kb = (double*)(base + w.kb);
+  double* vb = (double*)(base + w.vb);
+  double* at = (double*)(base + w.at);
+  double* kbb = (double*)(base + w.kbb);
+  double* sg = (double*)(base + w.sg);
+  double* arow = (double*)(base + w.arow);
+  double* ka = (double*)(base + w.ka);
+  const int64_t np = m->n_pad, nbp = w.nbp;
+  const int n_b = st->n_b;
+  int rc = covar(m->kind, st->xb_s, n_b, m->train_xs, m->n, m->d, m->outputscale, 0.0, false, kb,
+                 np, nbp, np, s);
+  if (rc) return rc;
+  rc = gemm(BTA_GEMM_B_LOWER, nbp, np, np, 1.0, kb, np, m->linv, np, 0.0, nullptr, 0, vb, np, s);
+  if (rc) return rc;
+  rc = gemm(BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, nbp, np, np, 1.0, vb, np, m->linv, np, 0.0,
+            nullptr, 0, at, np, s);
+  if (rc) return rc;
+  rc = cast_pad<double>(m->alpha, np, 1, np, arow, np, 64, np, s);
+  if (rc) return rc;
+  rc = gemm(0, nbp, 64, np, 1.0, kb, np, arow, np, 0.0, nullptr, 0, ka, 64, s);
+  if (rc) return rc;
+  // Sigma_bb = y_std^2 (k(Xb,Xb) - Vb Vb^T), identity-padded for the factorisation
+  rc = covar(m->kind, st->xb_s, n_b, st->xb_s, n_b, m->d, m->outputscale, 0.0, false, kbb, nbp,
+             nbp, nbp, s, /*pad_identity=*/1);
+  if (rc) return rc;
+  const double v2 = m->y_std * m->y_std;
+  rc = gemm(0, nbp, nbp, np, -v2, vb, np, vb, np, v2, kbb, nbp, sg, nbp, s);
+  if (rc) return rc;
+  rc = potrf(sg, nbp, nbp, nullptr, 0, st->max_tries, st->jitter, jitter_used, base + w.fact,
+             potrf_ws_bytes(nbp), s);
+  if (rc) return rc;
+  rc = cast_pad<double>(sg, nbp, n_b, n_b, Lb, n_b, n_b, n_b, s);
+  if (rc) return rc;
+  // mu_b (original scale) into the workspace, then best_f
+  double* mu = (double*)(base + w.mu);
+  {
+    // mu[j] = y_mean + y_std (c + ka[j*64])
+    rc = cast_pad<double>(ka, 64, nbp, 1, mu, 1, nbp, 1, s);
+    if (rc) return rc;
+  }
+  rc = launch_baseline_best_f(mu, m->y_mean, m->y_std, m->mean_constant, Lb, Zb, n_b, st->S,
+                              best_f, s);
+  if (rc) return rc;
+  // [L^-1; alpha^T; A^T; 0] in the state's precision
+  const int64_t rows = np + st->ext_pad;
+  if (st->precision == BTA_PREC_MIXED) {
+    float* E = (float*)linv_ext;
+    rc = cast_pad<float>(m->linv, np, np, np, E, np, np, np, s);
+    if (!rc) rc = cast_pad<float>(m->alpha, np, 1, np, E + np * np, np, 1, np, s);
+    if (!rc) rc = cast_pad<float>(at, np, n_b, np, E + (np + 1) * np, np, st->ext_pad - 1, np, s);
+  } else {
+    double* E = (double*)linv_ext;
+    rc = cast_pad<double>(m->linv, np, np, np, E, np, np, np, s);
+    if (!rc) rc = cast_pad<double>(m->alpha, np, 1, np, E + np * np, np, 1, np, s);
+    if (!rc)
+      rc = cast_pad<double>(at, np, n_b, np, E + (np + 1) * np, np, st->ext_pad - 1, np, s);
+  }
+  (void)rows;
+  return rc;
+}
+
+}  // namespace bta
+
+extern "C" size_t bta_gp_cache_workspace_size(int64_t n_pad) { return bta::gp_cache_ws(n_pad); }
+
+extern "C" int bta_gp_cache_build(const bta_gp_model* m, const double* y, double* L,
+                                  double* Linv, double* alpha, int32_t max_tries,
+                                  double* jitter_used, void* ws, size_t ws_bytes,
+                                  bta_stream_t stream) {
+  return bta::gp_cache_build(m, y, L, Linv, alpha, max_tries, jitter_used, ws, ws_bytes,
+                             (hipStream_t)stream);
+}
+
+extern "C" size_t bta_gp_posterior_workspace_size(const bta_gp_model* m, int64_t batch,
+                                                  int32_t mq) {
+  if (!m || batch < 1 || mq < 1) return 0;
+  return bta::PostWs(m, batch, mq).total;
+}
+
+extern "C" int bta_gp_posterior(const bta_gp_model* m, const double* X, int64_t batch,
+                                int32_t mq, double* mean, double* covar, double* var, void* ws,
+                                size_t ws_bytes, bta_stream_t stream) {
+  return bta::gp_posterior(m, X, batch, mq, mean, covar, var, ws, ws_bytes,
+                           (hipStream_t)stream);
+}
+
+extern "C" size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int32_t n_b,
+                                                     int32_t S) {
+  if (!m || n_b < 1 || S < 1) return 0;
+  return bta::PrepWs(m, n_b, S).total;
+}
+
+extern "C" int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
+                                   const double* Zb, void* linv_ext, double* Lb, double* best_f,
+                                   double* jitter_used, void* ws, size_t ws_bytes,
+                                   bta_stream_t stream) {
+  return bta::qlognei_prepare(m, st, Zb, linv_ext, Lb, best_f, jitter_used, ws, ws_bytes,
+                              (hipStream_t)stream);
+}

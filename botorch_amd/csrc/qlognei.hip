@@ -1,0 +1,679 @@
+// qLogNEI forward + backward, fused per restart.
+//
+// The posterior contraction V = K(X, X_train) L^-T (and, through the rows
+// appended to L^-1, K(X,X_train) alpha and K(X,X_train) K^-1 K(X_train,X_b))
+// comes from the MFMA GEMM.  Everything else for one restart b is small and
+// done by one 256-thread workgroup in LDS (qlognei_mc_kernel):
+//
+//   Sigma_qq = s^2 (K(X,X) - V V^T)        Sigma_qb = s^2 (K(X,X_b) - K_x A)
+//   mu       = y_mean + s (c + K_x alpha)                       (s = y_std)
+//   Lbl = Sigma_qb Lb^-T;  Lbr = chol(Sigma_qq - Lbl Lbl^T)  (jitter escalation)
+//   Y[s,i] = mu_i + [Lbl | Lbr]_i . Z[s]                       (cached root)
+//   v = log_fatplus(Y - best_f, tau_relu); u = fatmax_q(v, tau_max)
+//   value = logmeanexp_S(u)
+//
+// followed by the closed-form adjoint of every step (softmax weights of the
+// logmeanexp, the fat-max Pareto derivative with the amax tie split, the
+// fat-softplus derivative, Z-contractions, the q x q Cholesky backward, the
+// triangular solve with Lb), producing
+//   dVt_ext[r] = [ -2 s^2 (gA V)_r | s dmu_r | -s^2 dSigma_qb[r] ]   for the
+// adjoint GEMM with [L^-1; alpha^T; A^T], and the direct input gradient from
+// K(X,X) and K(X,X_b).  References: utils/low_rank.py:84-172,
+// acquisition/logei.py:376-391,688-715, utils/safe_math.py:146-191,298-355,
+// acquisition/monte_carlo.py:268-290.
+#include "bta_common.h"
+#include "gemm.h"
+#include "kernels.h"
+
+namespace bta {
+
+constexpr int MC_MAXD = 32;
+constexpr int MC_MAXNB_SLOTS = 4;  // n_b <= 256
+
+struct McArgs {
+  const void* vt;
+  int64_t ldv;
+  void* dvt;
+  int64_t lddv;
+  const double* xs;
+  const double* bs;
+  const double* Lb;
+  const double* Zt;
+  const double* best_f;
+  double* value;
+  double* dxs_extra;
+  int* status;
+  int d, n_pad, n_b, S, kind, max_tries, grad;
+  double outputscale, y_mean, y_std, c, tau_relu, tau_max, jitter;
+};
+
+// LDS layout (doubles) shared by host sizing and the kernel.
+struct McLayout {
+  int xq, red, mu, dmu, sqq, lbr, ga, lnew, dlnew, dsqb, u, G, scal, total;
+  __host__ __device__ McLayout(int Q, int n_b, int S) {
+    const int m = n_b + Q;
+    int o = 0;
+    xq = o; o += Q * MC_MAXD;
+    red = o; o += 4 * 64;
+    mu = o; o += Q;
+    dmu = o; o += Q;
+    sqq = o; o += Q * Q;
+    lbr = o; o += Q * Q;
+    ga = o; o += Q * Q;
+    lnew = o; o += Q * m;
+    dlnew = o; o += Q * m;
+    dsqb = o; o += Q * (n_b > 0 ? n_b : 1);
+    u = o; o += S;
+    G = o; o += S * Q;
+    scal = o; o += 8;
+    total = o;
+  }
+};
+
+__device__ __forceinline__ void log_fatplus_grad(double x, double tau, double* v, double* dv) {
+  // log(tau * (softplus(x/tau) + 0.1 / (1 + (x/tau)^2)))  utils/safe_math.py:298-325
+  const double z = x / tau;
+  double sp, sig;
+  if (z > 20.0) {  // torch softplus threshold
+    sp = z;
+    sig = 1.0;
+  } else {
+    const double e = exp(z);
+    sp = log1p(e);
+    sig = e / (1.0 + e);
+  }
+  const double cz = 1.0 / (1.0 + z * z);
+  const double f = sp + 0.1 * cz;
+  *v = log(tau * f);
+  *dv = (sig - 0.2 * z * cz * cz) / (f * tau);
+}
+
+// block-wide reduction of NV values (NV <= 64); every thread gets the result
+// in out[]; uses red (4*64 doubles).  All threads must call.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red, double (&out)[NV]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = wave_sum(v[i]);
+    if (lane == 0) red[wv * 64 + i] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) out[i] = red[i] + red[64 + i] + red[128 + i] + red[192 + i];
+  __syncthreads();
+}
+
+__device__ __forceinline__ double block_max1(double v, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double s = wave_max(v);
+  if (lane == 0) red[wv * 64] = s;
+  __syncthreads();
+  double r = fmax(fmax(red[0], red[64]), fmax(red[128], red[192]));
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double sq_dist(const double* a, const double* b, int d) {
+  double r2 = 0.0;
+  for (int j = 0; j < d; ++j) {
+    double df = a[j] - b[j];
+    r2 = fma(df, df, r2);
+  }
+  return r2;
+}
+
+// In-register Cholesky of a Q x Q SPD matrix (row-major, lower result).
+template <int Q>
+__device__ bool small_chol(const double* A, double jit, double* L) {
+  for (int i = 0; i < Q * Q; ++i) L[i] = 0.0;
+  for (int j = 0; j < Q; ++j) {
+    double s = A[j * Q + j] + jit;
+    for (int k = 0; k < j; ++k) s -= L[j * Q + k] * L[j * Q + k];
+    if (!(s > 0.0)) return false;
+    const double dj = sqrt(s);
+    L[j * Q + j] = dj;
+    for (int i = j + 1; i < Q; ++i) {
+      double t = A[i * Q + j];
+      for (int k = 0; k < j; ++k) t -= L[i * Q + k] * L[j * Q + k];
+      L[i * Q + j] = t / dj;
+    }
+  }
+  return true;
+}
+
+template <int Q, typename TV>
+__global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
+  extern __shared__ double sm[];
+  const McLayout lay(Q, a.n_b, a.S);
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int d = a.d, n_b = a.n_b, S = a.S, m = n_b + Q;
+  const double ys = a.y_std, ys2 = a.y_std * a.y_std;
+  double* xq = sm + lay.xq;
+  double* red = sm + lay.red;
+  double* mu = sm + lay.mu;
+  double* dmu = sm + lay.dmu;
+  double* sqq = sm + lay.sqq;
+  double* lbr = sm + lay.lbr;
+  double* gA = sm + lay.ga;
+  double* lnew = sm + lay.lnew;
+  double* dlnew = sm + lay.dlnew;
+  double* dsqb = sm + lay.dsqb;
+  double* uu = sm + lay.u;
+  double* G = sm + lay.G;
+  double* scal = sm + lay.scal;
+  const TV* vt = (const TV*)a.vt;
+  const int64_t r0 = (int64_t)b * Q;
+
+  for (int i = tid; i < Q * d; i += 256) xq[(i / d) * MC_MAXD + i % d] = a.xs[(r0 + i / d) * d + i % d];
+  if (tid == 0) scal[1] = 0.0;
+
+  // ---- Sigma_qq accumulation: V_b V_b^T over the n_pad columns -----------
+  constexpr int NQ = Q * (Q + 1) / 2;
+  {
+    double acc[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) acc[i] = 0.0;
+    for (int k = tid; k < a.n_pad; k += 256) {
+      double v[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) v[i] = (double)vt[(r0 + i) * a.ldv + k];
+      int idx = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+          acc[idx] = fma(v[i], v[j], acc[idx]);
+          ++idx;
+        }
+    }
+    double tot[NQ];
+    block_sum<NQ>(acc, red, tot);
+    if (tid < Q * Q) {
+      const int i = tid / Q, j = tid % Q;
+      const int ii = i > j ? i : j, jj = i > j ? j : i;
+      const double vv = tot[ii * (ii + 1) / 2 + jj];
+      const double kqq = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d),
+                                    a.outputscale);
+      sqq[i * Q + j] = ys2 * (kqq - vv);
+    }
+    if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + (double)vt[(r0 + tid) * a.ldv + a.n_pad]);
+  }
+  // ---- Sigma_qb into lnew[:, :n_b] ----------------------------------------
+  for (int e = tid; e < Q * n_b; e += 256) {
+    const int i = e / n_b, kb = e % n_b;
+    const double kqb = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d),
+                                  a.outputscale);
+    lnew[i * m + kb] = ys2 * (kqb - (double)vt[(r0 + i) * a.ldv + a.n_pad + 1 + kb]);
+  }
+  __syncthreads();
+
+  // ---- Lbl = Sigma_qb Lb^-T: forward substitution, one wave per row ------
+  for (int i = wv; i < Q; i += 4) {
+    double rhs[MC_MAXNB_SLOTS];
+#pragma unroll
+    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+      const int j = lane + 64 * t;
+      rhs[t] = j < n_b ? lnew[i * m + j] : 0.0;
+    }
+    for (int kb = 0; kb < n_b; ++kb) {
+      const int slot = kb >> 6, owner = kb & 63;
+      double mine = 0.0;
+#pragma unroll
+      for (int t = 0; t < MC_MAXNB_SLOTS; ++t)
+        if (t == slot) mine = rhs[t];
+      const double y = __shfl(mine, owner, 64) / a.Lb[kb * n_b + kb];
+#pragma unroll
+      for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+        const int j = lane + 64 * t;
+        if (j == kb) rhs[t] = y;
+        else if (j > kb && j < n_b) rhs[t] -= a.Lb[j * n_b + kb] * y;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+      const int j = lane + 64 * t;
+      if (j < n_b) lnew[i * m + j] = rhs[t];
+    }
+  }
+  __syncthreads();
+
+  // ---- Schur complement and its Cholesky (jitter escalation) -------------
+  if (tid < Q * Q) {
+    const int i = tid / Q, j = tid % Q;
+    double s = 0.0;
+    for (int kb = 0; kb < n_b; ++kb) s = fma(lnew[i * m + kb], lnew[j * m + kb], s);
+    gA[i * Q + j] = sqq[i * Q + j] - s;  // gA used as scratch for the Schur block
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int st = 0;
+    bool ok = false;
+    bool has_nan = false;
+    for (int i = 0; i < Q * Q; ++i) has_nan |= (gA[i] != gA[i]);
+    for (int attempt = 0; attempt <= a.max_tries && !has_nan; ++attempt) {
+      const double jit = attempt == 0 ? 0.0 : a.jitter * pow(10.0, (double)(attempt - 1));
+      ok = small_chol<Q>(gA, jit, lbr);
+      if (ok) {
+        if (attempt > 0) st |= BTA_STATUS_JITTER;
+        break;
+      }
+    }
+    if (has_nan) st |= BTA_STATUS_NAN;
+    else if (!ok) st |= BTA_STATUS_NOT_PSD;
+    if (!ok)
+      for (int i = 0; i < Q * Q; ++i) lbr[i] = NAN;
+    scal[0] = (double)st;
+  }
+  __syncthreads();
+  for (int e = tid; e < Q * Q; e += 256) {
+    const int i = e / Q, j = e % Q;
+    lnew[i * m + n_b + j] = lbr[i * Q + j];
+  }
+  __syncthreads();
+
+  // ---- samples, log improvement, fat-max over q ---------------------------
+  const double tr = a.tau_relu, tm = a.tau_max;
+  bool bad = false;
+  for (int s = tid; s < S; s += 256) {
+    double y[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) y[i] = mu[i];
+    for (int k = 0; k < m; ++k) {
+      const double z = a.Zt[(int64_t)k * S + s];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) y[i] = fma(lnew[i * m + k], z, y[i]);
+    }
+    const double bf = a.best_f[s];
+    double v[Q], dv[Q];
+    double M = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      bad |= !isfinite(y[i]);
+      log_fatplus_grad(y[i] - bf, tr, &v[i], &dv[i]);
+      M = fmax(M, v[i]);
+    }
+    double P = 0.0, sdp = 0.0, dp[Q];
+    int nties = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const double z = (M - v[i]) / tm;
+      const double den = 2.0 + 2.0 * z + z * z;
+      P += 2.0 / den;
+      dp[i] = -2.0 * (2.0 + 2.0 * z) / (den * den);
+      sdp += dp[i];
+      nties += (v[i] == M);
+    }
+    uu[s] = M + tm * log(P);
+    const double cm = (1.0 + sdp / P) / (double)nties;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const double du = (v[i] == M ? cm : 0.0) - dp[i] / P;
+      G[s * Q + i] = du * dv[i];
+    }
+  }
+  if (bad) scal[1] = 1.0;  // every writer stores the same value
+  __syncthreads();
+
+  // ---- logmeanexp over samples --------------------------------------------
+  double lmax = -INFINITY;
+  for (int s = tid; s < S; s += 256) lmax = fmax(lmax, uu[s]);
+  const double Mu = block_max1(lmax, red);
+  double lsum[1] = {0.0};
+  for (int s = tid; s < S; s += 256) lsum[0] += exp(uu[s] - Mu);
+  double tsum[1];
+  block_sum<1>(lsum, red, tsum);
+  if (tid == 0) {
+    a.value[b] = Mu + log(tsum[0]) - log((double)S);
+    int st = (int)scal[0];
+    if (scal[1] != 0.0) st |= BTA_STATUS_NAN;
+    a.status[b] = st;
+  }
+  if (!a.grad) return;
+
+  for (int s = tid; s < S; s += 256) {
+    const double w = exp(uu[s] - Mu) / tsum[0];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) G[s * Q + i] *= w;
+  }
+  __syncthreads();
+
+  // ---- dmu, d[Lbl | Lbr] = G^T Z: one wave per (i, k) --------------------
+  for (int task = wv; task < Q * (m + 1); task += 4) {
+    const int i = task / (m + 1), k = task % (m + 1);
+    if (k < m && k >= n_b + i + 1) {  // upper part of Lbr: no dependence
+      if (lane == 0) dlnew[i * m + k] = 0.0;
+      continue;
+    }
+    double acc = 0.0;
+    if (k == m) {
+      for (int s = lane; s < S; s += 64) acc += G[s * Q + i];
+    } else {
+      const double* zr = a.Zt + (int64_t)k * S;
+      for (int s = lane; s < S; s += 64) acc = fma(G[s * Q + i], zr[s], acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      if (k == m) dmu[i] = acc;
+      else dlnew[i * m + k] = acc;
+    }
+  }
+  __syncthreads();
+
+  // ---- Cholesky backward of Lbr (torch's formula), gA symmetric ----------
+  if (tid == 0) {
+    double T[Q * Q], Mm[Q * Q], Y[Q * Q];
+    for (int i = 0; i < Q; ++i)
+      for (int j = 0; j < Q; ++j) {
+        double t = 0.0;
+        if (j <= i)
+          for (int k = 0; k < Q; ++k) t += lbr[k * Q + i] * dlnew[k * m + n_b + j];
+        T[i * Q + j] = t;  // tril(L^T dL)
+      }
+    for (int i = 0; i < Q; ++i)
+      for (int j = 0; j < Q; ++j)
+        Mm[i * Q + j] = 0.5 * (i >= j ? T[i * Q + j] : T[j * Q + i]);
+    // Y = L^-T Mm : solve L^T Y = Mm (upper, back substitution per column)
+    for (int j = 0; j < Q; ++j)
+      for (int i = Q - 1; i >= 0; --i) {
+        double s = Mm[i * Q + j];
+        for (int k = i + 1; k < Q; ++k) s -= lbr[k * Q + i] * Y[k * Q + j];
+        Y[i * Q + j] = s / lbr[i * Q + i];
+      }
+    // gA = Y L^-1 : solve gA L = Y, i.e. for each row r: x L = y
+    for (int r = 0; r < Q; ++r)
+      for (int j = Q - 1; j >= 0; --j) {
+        double s = Y[r * Q + j];
+        for (int k = j + 1; k < Q; ++k) s -= gA[r * Q + k] * lbr[k * Q + j];
+        gA[r * Q + j] = s / lbr[j * Q + j];
+      }
+  }
+  __syncthreads();
+  // dLbl -= 2 gA Lbl  (Schur = Sigma_qq - Lbl Lbl^T)
+  for (int e = tid; e < Q * n_b; e += 256) {
+    const int i = e / n_b, kb = e % n_b;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) s += gA[i * Q + j] * lnew[j * m + kb];
+    dlnew[i * m + kb] -= 2.0 * s;
+  }
+  __syncthreads();
+  // ---- dSigma_qb = dLbl Lb^-1 : back substitution, one wave per row -------
+  for (int i = wv; i < Q; i += 4) {
+    double rhs[MC_MAXNB_SLOTS];
+#pragma unroll
+    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+      const int j = lane + 64 * t;
+      rhs[t] = j < n_b ? dlnew[i * m + j] : 0.0;
+    }
+    for (int kb = n_b - 1; kb >= 0; --kb) {
+      const int slot = kb >> 6, owner = kb & 63;
+      double mine = 0.0;
+#pragma unroll
+      for (int t = 0; t < MC_MAXNB_SLOTS; ++t)
+        if (t == slot) mine = rhs[t];
+      const double zz = __shfl(mine, owner, 64) / a.Lb[kb * n_b + kb];
+#pragma unroll
+      for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+        const int j = lane + 64 * t;
+        if (j == kb) rhs[t] = zz;
+        else if (j < kb) rhs[t] -= a.Lb[kb * n_b + j] * zz;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+      const int j = lane + 64 * t;
+      if (j < n_b) dsqb[i * n_b + j] = rhs[t];
+    }
+  }
+  __syncthreads();
+
+  // ---- dVt_ext rows -------------------------------------------------------
+  TV* dvt = (TV*)a.dvt;
+  for (int k = tid; k < a.n_pad; k += 256) {
+    double v[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) v[j] = (double)vt[(r0 + j) * a.ldv + k];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < Q; ++j) s = fma(gA[i * Q + j], v[j], s);
+      dvt[(r0 + i) * a.lddv + k] = (TV)(-2.0 * ys2 * s);
+    }
+  }
+  const int next = (int)(a.lddv - a.n_pad);
+  for (int e = tid; e < Q * next; e += 256) {
+    const int i = e / next, k = e % next;
+    double val = 0.0;
+    if (k == 0) val = ys * dmu[i];
+    else if (k <= n_b) val = -ys2 * dsqb[i * n_b + k - 1];
+    dvt[(r0 + i) * a.lddv + a.n_pad + k] = (TV)val;
+  }
+  // ---- direct input gradient through K(X, X) and K(X, X_b) (scaled) ------
+  for (int e = tid; e < Q * d; e += 256) {
+    const int i = e / d, j = e % d;
+    const double* xi = xq + i * MC_MAXD;
+    double acc = 0.0;
+    for (int i2 = 0; i2 < Q; ++i2) {
+      if (i2 == i) continue;
+      const double* x2 = xq + i2 * MC_MAXD;
+      const double h = kern_grad_factor(a.kind, sq_dist(xi, x2, d), a.outputscale, nullptr);
+      acc -= 2.0 * ys2 * gA[i * Q + i2] * h * (xi[j] - x2[j]);
+    }
+    for (int kb = 0; kb < n_b; ++kb) {
+      const double* x2 = a.bs + kb * d;
+      const double h = kern_grad_factor(a.kind, sq_dist(xi, x2, d), a.outputscale, nullptr);
+      acc -= ys2 * dsqb[i * n_b + kb] * h * (xi[j] - x2[j]);
+    }
+    a.dxs_extra[(r0 + i) * d + j] = acc;
+  }
+}
+
+template <int Q>
+static int launch_mc_q(const McArgs& a, int64_t b, bool f32, hipStream_t s) {
+  const McLayout lay(Q, a.n_b, a.S);
+  const size_t bytes = (size_t)lay.total * sizeof(double);
+  if (bytes > 64 * 1024) {
+    set_error("qlognei: LDS need %zu bytes exceeds 64 KiB (q=%d n_b=%d S=%d)", bytes, Q, a.n_b,
+              a.S);
+    return BTA_ERR_UNSUPPORTED;
+  }
+  if (f32)
+    hipLaunchKernelGGL((qlognei_mc_kernel<Q, float>), dim3((unsigned)b), dim3(256), bytes, s, a);
+  else
+    hipLaunchKernelGGL((qlognei_mc_kernel<Q, double>), dim3((unsigned)b), dim3(256), bytes, s,
+                       a);
+  BTA_LAUNCH_CHECK("qlognei_mc_kernel");
+  return BTA_OK;
+}
+
+static int launch_mc(const McArgs& a, int q, int64_t b, bool f32, hipStream_t s) {
+  switch (q) {
+    case 1: return launch_mc_q<1>(a, b, f32, s);
+    case 2: return launch_mc_q<2>(a, b, f32, s);
+    case 3: return launch_mc_q<3>(a, b, f32, s);
+    case 4: return launch_mc_q<4>(a, b, f32, s);
+    case 5: return launch_mc_q<5>(a, b, f32, s);
+    case 6: return launch_mc_q<6>(a, b, f32, s);
+    case 7: return launch_mc_q<7>(a, b, f32, s);
+    case 8: return launch_mc_q<8>(a, b, f32, s);
+  }
+  set_error("qlognei: q must be in 1..8 (got %d)", q);
+  return BTA_ERR_UNSUPPORTED;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace plan for one value+grad call.
+// ---------------------------------------------------------------------------
+struct QlWs {
+  int64_t R, R_pad, ldv;
+  size_t xs, kx, vt, dvt, dxe, total;
+  QlWs(const bta_gp_model* m, const bta_qlognei_state* st, int64_t b) {
+    const size_t es = st->precision == BTA_PREC_MIXED ? 4 : 8;
+    R = b * st->q;
+    R_pad = round_up(R > 0 ? R : 1, 128);
+    ldv = m->n_pad + st->ext_pad;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+      size_t r = o;
+      o += round_up((int64_t)bytes, 256);
+      return r;
+    };
+    xs = take((size_t)R_pad * m->d * 8);
+    kx = take((size_t)R_pad * m->n_pad * es);
+    vt = take((size_t)R_pad * ldv * es);
+    dvt = take((size_t)R_pad * ldv * es);
+    dxe = take((size_t)R_pad * m->d * 8);
+    total = o;
+  }
+};
+
+int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const double* X,
+                       int64_t b, double* value, double* grad, int* status, void* ws,
+                       size_t ws_bytes, hipStream_t s) {
+  if (!m || !st || !X || !value || !status || b < 1 || b > (1 << 24)) {
+    set_error("qlognei_value_grad: invalid arguments");
+    return BTA_ERR_INVALID;
+  }
+  if (st->n_b < 1 || st->n_b > 64 * MC_MAXNB_SLOTS || st->S < 1 || st->q < 1 || st->q > 8 ||
+      st->ext_pad < st->n_b + 1 || st->ext_pad % 128 || m->n_pad % 128 || m->d > MC_MAXD) {
+    set_error("qlognei_value_grad: unsupported state (n_b=%d S=%d q=%d ext_pad=%ld)", st->n_b,
+              st->S, st->q, (long)st->ext_pad);
+    return BTA_ERR_UNSUPPORTED;
+  }
+  const QlWs w(m, st, b);
+  if (!ws || ws_bytes < w.total) {
+    set_error("qlognei_value_grad: workspace %zu < %zu", ws_bytes, w.total);
+    return BTA_ERR_WORKSPACE;
+  }
+  char* base = (char*)ws;
+  double* xs = (double*)(base + w.xs);
+  void* kx = base + w.kx;
+  void* vt = base + w.vt;
+  void* dvt = base + w.dvt;
+  double* dxe = (double*)(base + w.dxe);
+  const bool f32 = st->precision == BTA_PREC_MIXED;
+  const int fl = f32 ? BTA_GEMM_F32 : 0;
+  int rc = scale_inputs(X, w.R, m->d, m->center, m->lengthscale, nullptr, xs, w.R_pad, s);
+  if (rc) return rc;
+  rc = covar(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, 0.0, f32, kx, m->n_pad,
+             w.R_pad, m->n_pad, s);
+  if (rc) return rc;
+  // [V | K_x alpha | K_x A] = K_x [L^-1; alpha^T; A^T]^T
+  rc = gemm(fl | BTA_GEMM_B_LOWER, w.R_pad, w.ldv, m->n_pad, 1.0, kx, m->n_pad, st->linv_ext,
+            m->n_pad, 0.0, nullptr, 0, vt, w.ldv, s);
+  if (rc) return rc;
+  McArgs a;
+  a.vt = vt;
+  a.ldv = w.ldv;
+  a.dvt = dvt;
+  a.lddv = w.ldv;
+  a.xs = xs;
+  a.bs = st->xb_s;
+  a.Lb = st->Lb;
+  a.Zt = st->Zt;
+  a.best_f = st->best_f;
+  a.value = value;
+  a.dxs_extra = dxe;
+  a.status = status;
+  a.d = m->d;
+  a.n_pad = (int)m->n_pad;
+  a.n_b = st->n_b;
+  a.S = st->S;
+  a.kind = m->kind;
+  a.max_tries = st->max_tries;
+  a.grad = grad != nullptr;
+  a.outputscale = m->outputscale;
+  a.y_mean = m->y_mean;
+  a.y_std = m->y_std;
+  a.c = m->mean_constant;
+  a.tau_relu = st->tau_relu;
+  a.tau_max = st->tau_max;
+  a.jitter = st->jitter;
+  rc = launch_mc(a, st->q, b, f32, s);
+  if (rc || !grad) return rc;
+  // dK_x = dVt_ext [L^-1; alpha^T; A^T]   (K x N layout of the same matrix)
+  rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, m->n_pad, w.ldv, 1.0, dvt, w.ldv,
+            st->linv_ext, m->n_pad, 0.0, nullptr, 0, kx, m->n_pad, s);
+  if (rc) return rc;
+  return covar_grad(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, kx, f32,
+                    m->n_pad, dxe, m->lengthscale, grad, s);
+}
+
+// ---------------------------------------------------------------------------
+// Baseline preparation helpers
+// ---------------------------------------------------------------------------
+// best_f[s] = max_j (mu_b[j] + sum_k Lb[j][k] Zb[s][k]) with
+// mu_b = y_mean + y_std (c + ka)   (acquisition/utils.py:134-138, logei.py:431-458)
+__global__ void baseline_best_f_kernel(const double* ka, double y_mean, double y_std, double c,
+                                       const double* Lb, const double* Zb, int n_b, int S,
+                                       double* best_f) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double best = -INFINITY;
+  for (int j = 0; j < n_b; ++j) {
+    double y = y_mean + y_std * (c + ka[j]);
+    for (int k = 0; k <= j; ++k) y = fma(Lb[j * n_b + k], Zb[(int64_t)s * n_b + k], y);
+    best = fmax(best, y);
+  }
+  best_f[s] = best;
+}
+
+// out[i][j] = (TO) in[i][j] for a rows x cols block, zero elsewhere up to the
+// padded shape.
+template <typename TO>
+__global__ void cast_pad_kernel(const double* in, int64_t ldi, int64_t rows, int64_t cols,
+                                TO* out, int64_t ldo, int64_t rows_pad, int64_t cols_pad) {
+  const int64_t i = blockIdx.y;
+  if (i >= rows_pad) return;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cols_pad;
+       j += (int64_t)gridDim.x * blockDim.x)
+    out[i * ldo + j] = (i < rows && j < cols) ? (TO)in[i * ldi + j] : (TO)0;
+}
+
+template <typename TO>
+int cast_pad(const double* in, int64_t ldi, int64_t rows, int64_t cols, TO* out, int64_t ldo,
+             int64_t rows_pad, int64_t cols_pad, hipStream_t s) {
+  if (rows_pad <= 0 || cols_pad <= 0) return BTA_OK;
+  dim3 g((unsigned)((cols_pad + 255) / 256 > 32 ? 32 : (cols_pad + 255) / 256),
+         (unsigned)rows_pad);
+  hipLaunchKernelGGL((cast_pad_kernel<TO>), g, dim3(256), 0, s, in, ldi, rows, cols, out, ldo,
+                     rows_pad, cols_pad);
+  BTA_LAUNCH_CHECK("cast_pad_kernel");
+  return BTA_OK;
+}
+template int cast_pad<double>(const double*, int64_t, int64_t, int64_t, double*, int64_t,
+                              int64_t, int64_t, hipStream_t);
+template int cast_pad<float>(const double*, int64_t, int64_t, int64_t, float*, int64_t, int64_t,
+                             int64_t, hipStream_t);
+
+int launch_baseline_best_f(const double* ka, double y_mean, double y_std, double c,
+                           const double* Lb, const double* Zb, int n_b, int S, double* best_f,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(baseline_best_f_kernel, dim3((S + 255) / 256), dim3(256), 0, s, ka, y_mean,
+                     y_std, c, Lb, Zb, n_b, S, best_f);
+  BTA_LAUNCH_CHECK("baseline_best_f_kernel");
+  return BTA_OK;
+}
+
+size_t qlognei_ws_bytes(const bta_gp_model* m, const bta_qlognei_state* st, int64_t b) {
+  return QlWs(m, st, b).total;
+}
+
+}  // namespace bta
+
+extern "C" size_t bta_qlognei_workspace_size(const bta_gp_model* m, const bta_qlognei_state* st,
+                                             int64_t b) {
+  if (!m || !st || b < 1) return 0;
+  return bta::qlognei_ws_bytes(m, st, b);
+}
+
+extern "C" int bta_qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st,
+                                      const double* X, int64_t b, double* value, double* grad,
+                                      int32_t* status, void* ws, size_t ws_bytes,
+                                      bta_stream_t stream) {
+  return bta::qlognei_value_grad(m, st, X, b, value, grad, status, ws, ws_bytes,
+                                 (hipStream_t)stream);
+}

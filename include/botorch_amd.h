@@ -1,0 +1,250 @@
+/*
+ * botorch_amd C ABI - MI355X (gfx950) kernels for BoTorch's GP-posterior /
+ * cached-root Sobol sampling / qLogNEI hot path and the exact-MLL gradient.
+ *
+ * Conventions
+ *  - Every pointer argument is DEVICE memory unless its comment says "host".
+ *  - Matrices are row-major with an explicit leading dimension (elements).
+ *  - "Scaled" inputs are (x - center) / lengthscale, coordinate-wise.
+ *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = default
+ *    stream) unless its comment says it synchronises; it allocates no device
+ *    memory: scratch comes from the caller through (ws, ws_bytes) sized by the
+ *    matching *_workspace_size function.
+ *  - Return value: BTA_OK or a BTA_ERR_* code; bta_last_error() gives the text
+ *    (thread-local).
+ *
+ * Each entry point names the reference interface (botorch 0.18.1 file:line) it
+ * stands in for.  The reference's own arithmetic sits in gpytorch /
+ * linear_operator, reached from the cited call sites.
+ */
+#ifndef BOTORCH_AMD_H
+#define BOTORCH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BTA_ABI_VERSION 1
+
+enum {
+  BTA_OK = 0,
+  BTA_ERR_INVALID = 1,     /* bad argument / shape */
+  BTA_ERR_NOT_PSD = 2,     /* Cholesky failed after all jitter tries (NotPSDError) */
+  BTA_ERR_NAN = 3,         /* NaN input to a factorisation (NanError) */
+  BTA_ERR_UNSUPPORTED = 4, /* shape outside the implemented envelope (UnsupportedError) */
+  BTA_ERR_HIP = 5,         /* HIP runtime error */
+  BTA_ERR_WORKSPACE = 6    /* workspace too small */
+};
+
+enum { BTA_KERNEL_MATERN52 = 0, BTA_KERNEL_RBF = 1 };
+enum { BTA_PREC_FP64 = 0, BTA_PREC_MIXED = 1 };
+
+/* per-restart status bits written by bta_qlognei_value_grad */
+enum { BTA_STATUS_JITTER = 1, BTA_STATUS_NOT_PSD = 2, BTA_STATUS_NAN = 4 };
+
+/* bta_gemm flags */
+enum {
+  BTA_GEMM_A_LOWER = 1,    /* A[m][k] == 0 for k > m            */
+  BTA_GEMM_A_UPPER = 2,    /* A[m][k] == 0 for k < m            */
+  BTA_GEMM_B_LOWER = 4,    /* element (n,k) of B == 0 for k > n */
+  BTA_GEMM_B_UPPER = 8,    /* element (n,k) of B == 0 for k < n */
+  BTA_GEMM_C_LOWER = 16,   /* only output tiles touching the lower triangle */
+  BTA_GEMM_B_KN = 32,      /* B stored K x N row-major (else N x K row-major) */
+  BTA_GEMM_F32 = 64,       /* A, B, C, C0 are float (else double) */
+  BTA_GEMM_A_KM = 128      /* A stored K x M row-major (else M x K row-major) */
+};
+
+typedef void* bta_stream_t; /* hipStream_t */
+
+const char* bta_version(void);
+int bta_abi_version(void);
+const char* bta_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Covariance construction.                                                  */
+/* Replaces gpytorch MaternKernel(nu=2.5) / RBFKernel / ScaleKernel forward,  */
+/* reached from models/gp_regression.py:209-214 (SingleTaskGP.forward) and   */
+/* models/utils/gpytorch_modules.py:121-133.                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Xs[i][j] = (X[i][j] - center[j]) / lengthscale[j] for i < n, 0 for n <= i <
+ * n_pad.  If center is NULL the column means of X are used and written to
+ * center_out (d doubles) when that is not NULL. */
+int bta_scale_inputs(const double* X, int64_t n, int32_t d, const double* center,
+                     const double* lengthscale, double* center_out, double* Xs,
+                     int64_t n_pad, bta_stream_t stream);
+
+/* K[i][j] = outputscale * k(X1s[i], X2s[j]) (+ diag_add when i == j) for i <
+ * n1, j < n2; zero in the padding, except K[i][i] = 1 for n1 <= i < min(n1_pad,
+ * n2_pad) when diag_add != 0 (identity padding for factorisations).  K is
+ * double, or float when out_f32 != 0. */
+int bta_covar(int32_t kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2,
+              int32_t d, double outputscale, double diag_add, int32_t out_f32, void* K,
+              int64_t ldk, int64_t n1_pad, int64_t n2_pad, bta_stream_t stream);
+
+/* Adjoint of bta_covar w.r.t. X1 (unscaled coordinates):
+ * dX1[i][:] = (sum_j G[i][j] dk(X1s[i], X2s[j])/dX1s[i] + extra[i][:]) / lengthscale.
+ * G is double or float (g_f32); extra may be NULL. */
+int bta_covar_grad(int32_t kind, const double* X1s, int64_t n1, const double* X2s,
+                   int64_t n2, int32_t d, double outputscale, const void* G, int32_t g_f32,
+                   int64_t ldg, const double* extra, const double* lengthscale, double* dX1,
+                   bta_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Dense linear algebra (MFMA).                                              */
+/* ------------------------------------------------------------------------ */
+
+/* C = alpha * op(A) * op(B) + beta * C0 with op(A)[m][k] = A[m][k] (default)
+ * or A[k][m] (BTA_GEMM_A_KM), op(B)[k][n] = B[n][k] (default) or B[k][n]
+ * (BTA_GEMM_B_KN).  M, N, K must be multiples of 64.  batch copies via
+ * element strides sA, sB, sC0, sC.  Triangular flags skip zero K-ranges.
+ * C0 may alias C.  Stands in for the dense products torch issues inside
+ * linear_operator's MatmulLinearOperator / root decompositions. */
+int bta_gemm(int32_t flags, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+             int64_t lda, const void* B, int64_t ldb, double beta, const void* C0,
+             int64_t ldc0, void* C, int64_t ldc, int64_t batch, int64_t sA, int64_t sB,
+             int64_t sC0, int64_t sC, bta_stream_t stream);
+
+size_t bta_potrf_workspace_size(int64_t n_pad);
+
+/* In-place lower Cholesky of the n_pad x n_pad matrix A (n_pad a multiple of
+ * 64) with linear_operator psd_safe_cholesky semantics (reached from
+ * utils/low_rank.py:140, acquisition/cached_cholesky.py:123; max_tries=6 from
+ * botorch/__init__.py:54): factor; if it fails add jitter0*10^i to the
+ * diagonal (i = 0..max_tries-1) and refactor; BTA_ERR_NOT_PSD after the last
+ * try, BTA_ERR_NAN if A holds NaN.  Also writes the inverses of the 64x64
+ * diagonal blocks of L into the diagonal blocks of Linv (ld ldinv; may be NULL)
+ * and zeroes the strict upper triangle of A.  *jitter_used (host) receives the
+ * jitter that was added (0 if none).  SYNCHRONISES on `stream` (reads info),
+ * as the reference does on any(info). */
+int bta_potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv,
+              int32_t max_tries, double jitter0, double* jitter_used, void* ws,
+              size_t ws_bytes, bta_stream_t stream);
+
+/* Lower-triangular inverse: Linv = L^-1, where Linv already holds the inverted
+ * 64x64 diagonal blocks (from bta_potrf).  Recursive doubling over block
+ * pairs, two GEMMs per level.  Used for fast_pred_var's R = L^-T
+ * (gpytorch DefaultPredictionStrategy via models/gpytorch.py:585). */
+int bta_trtri_lower(const double* L, int64_t ldl, double* Linv, int64_t ldinv,
+                    int64_t n_pad, void* ws, size_t ws_bytes, bta_stream_t stream);
+
+/* Batched Cholesky of small n x n matrices (n <= 64) with per-matrix jitter
+ * escalation; status[b] gets BTA_STATUS_* bits.  Strict upper triangle zeroed. */
+int bta_chol_small_batched(double* A, int32_t n, int64_t lda, int64_t stride,
+                           int64_t batch, int32_t max_tries, double jitter0,
+                           int32_t* status, bta_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* GP model: SingleTaskGP exact prediction (fast_pred_var).                  */
+/* Replaces models/gpytorch.py:544-610 (BatchedMultiOutputGPyTorchModel.     */
+/* posterior) -> gpytorch ExactGP.__call__ / DefaultPredictionStrategy, with  */
+/* Standardize.untransform_posterior (models/transforms/outcome.py:431-511). */
+/* ------------------------------------------------------------------------ */
+typedef struct bta_gp_model {
+  int32_t kind;             /* BTA_KERNEL_* */
+  int32_t d;
+  int64_t n;                /* training points */
+  int64_t n_pad;            /* multiple of 128 */
+  const double* train_xs;   /* n_pad x d scaled training inputs (zero padding) */
+  const double* center;     /* d */
+  const double* lengthscale;/* d */
+  double outputscale;
+  double noise;             /* likelihood noise variance */
+  double mean_constant;     /* ConstantMean */
+  double y_mean, y_std;     /* Standardize statistics */
+  const double* linv;       /* n_pad x n_pad  L^-1 (lower, zero padding); set by caller */
+  const double* alpha;      /* n_pad  K^-1 (y_std - c) */
+} bta_gp_model;
+
+size_t bta_gp_cache_workspace_size(int64_t n_pad);
+
+/* Builds the prediction caches: K = k(X, X) + noise I, L = chol(K) with
+ * jitter, Linv = L^-1, alpha = Linv^T Linv (y - c).  y: n standardized
+ * targets.  L, Linv: n_pad x n_pad; alpha: n_pad.  SYNCHRONISES (potrf info). */
+int bta_gp_cache_build(const bta_gp_model* m, const double* y, double* L, double* Linv,
+                       double* alpha, int32_t max_tries, double* jitter_used, void* ws,
+                       size_t ws_bytes, bta_stream_t stream);
+
+size_t bta_gp_posterior_workspace_size(const bta_gp_model* m, int64_t batch, int32_t mq);
+
+/* Posterior over X (batch x mq x d, unscaled) in the ORIGINAL outcome scale:
+ * mean (batch x mq), covar (batch x mq x mq, may be NULL), var (batch x mq,
+ * may be NULL).  mean = y_mean + y_std (c + K*x alpha);
+ * covar = y_std^2 (K** - (K*x L^-T)(K*x L^-T)^T). */
+int bta_gp_posterior(const bta_gp_model* m, const double* X, int64_t batch, int32_t mq,
+                     double* mean, double* covar, double* var, void* ws, size_t ws_bytes,
+                     bta_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* qLogNoisyExpectedImprovement (cache_root=True, fat=True, incremental,     */
+/* IdentityMCObjective, no constraints).  Replaces the forward + autograd    */
+/* backward of acquisition/logei.py:527-565 (_get_samples_and_objectives),   */
+/* utils/low_rank.py:84-172 (sample_cached_cholesky), logei.py:376-391,688-  */
+/* 715 (_sample_forward / _log_improvement) and monte_carlo.py:268-290       */
+/* (fatmax over q, logmeanexp over samples), as called once per L-BFGS step  */
+/* by generation/gen.py:423-485 (f_np_wrapper).                              */
+/* ------------------------------------------------------------------------ */
+typedef struct bta_qlognei_state {
+  int32_t precision;        /* BTA_PREC_FP64 or BTA_PREC_MIXED */
+  int32_t q;                /* 1..8 */
+  int32_t n_b;              /* pruned baseline points */
+  int32_t S;                /* MC samples */
+  int64_t ext_pad;          /* rows appended to Linv: multiple of 128 >= 1 + n_b */
+  const double* xb_s;       /* n_b x d scaled baseline inputs */
+  const void* linv_ext;     /* (n_pad + ext_pad) x n_pad: [L^-1; alpha^T; A^T; 0],
+                               A = K^-1 k(X, X_b); double (FP64) or float (MIXED) */
+  const double* Lb;         /* n_b x n_b cached baseline Cholesky (original scale) */
+  const double* Zt;         /* (n_b + q) x S base samples, transposed */
+  const double* best_f;     /* S per-sample best baseline objective */
+  double tau_relu, tau_max;
+  int32_t max_tries;        /* psd_safe_cholesky tries for the q x q block */
+  double jitter;            /* 1e-8 (FP64 default) */
+} bta_qlognei_state;
+
+size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int32_t n_b, int32_t S);
+
+/* Builds linv_ext (given m->linv and m->alpha), Lb (psd_safe_cholesky of the
+ * baseline posterior covariance, original scale) and best_f (per-sample max of
+ * the baseline samples mu_b + Lb z_b, z_b = Zb (S x n_b)), as
+ * acquisition/logei.py:393-459 (_init_baseline) does.  linv_ext must hold
+ * (n_pad + ext_pad) x n_pad elements of the state's precision.  SYNCHRONISES. */
+int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
+                        const double* Zb, void* linv_ext, double* Lb, double* best_f,
+                        double* jitter_used, void* ws, size_t ws_bytes,
+                        bta_stream_t stream);
+
+size_t bta_qlognei_workspace_size(const bta_gp_model* m, const bta_qlognei_state* st,
+                                  int64_t b);
+
+/* value[b] = qLogNEI(X[b]) for X: b x q x d (unscaled); grad (b x q x d, may be
+ * NULL for value only) = d value[b] / d X[b].  status[b]: BTA_STATUS_* bits. */
+int bta_qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st,
+                           const double* X, int64_t b, double* value, double* grad,
+                           int32_t* status, void* ws, size_t ws_bytes,
+                           bta_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Exact marginal log likelihood value + adjoint.  Replaces gpytorch          */
+/* ExactMarginalLogLikelihood forward + torch autograd backward as driven by */
+/* optim/closures/model_closures.py:125-145 and fit.py:76-289.               */
+/* ------------------------------------------------------------------------ */
+size_t bta_mll_workspace_size(int64_t n, int32_t d);
+
+/* For K = outputscale k(X, X; lengthscale) + noise I and r = y - constant:
+ * out (host, 4 + d doubles) = [ log N(y; c, K)  (un-normalised, no priors),
+ *   d/d outputscale, d/d noise, d/d constant, d/d lengthscale[0..d) ].
+ * The gradient is 1/2 tr((a a^T - K^-1) dK/dtheta), a = K^-1 r.  X: n x d,
+ * y: n, lengthscale: host d doubles.  SYNCHRONISES. */
+int bta_mll_value_grad(int32_t kind, const double* X, const double* y, int64_t n, int32_t d,
+                       const double* lengthscale, double outputscale, double noise,
+                       double constant, int32_t max_tries, double* out, void* ws,
+                       size_t ws_bytes, bta_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOTORCH_AMD_H */
