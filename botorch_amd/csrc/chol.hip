@@ -41,11 +41,10 @@ __device__ int chol_lds(double* Ls, int n, int* nan_seen) {
       break;
     }
     const double dg = sqrt(p);
-    if (t < n) {
-      if (t > c) Ls[t * LP + c] /= dg;
-      else if (t == c) Ls[c * LP + c] = dg;
-    }
+    if (t > c && t < n) Ls[t * LP + c] /= dg;
     __syncthreads();
+    // every thread has read the pivot by now; nothing below reads L[c][c]
+    if (t == c) Ls[c * LP + c] = dg;
     const int i = c + 1 + (t >> 2);
     if (i < n) {
       const double lic = Ls[i * LP + c];
