@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       for (int ni = 0; ni < C::NI; ++ni)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int64_t row = tm * BM + wy * (BM / 2) + mi * 16 + gq * 4 + q;
+          const int64_t row = tm * BM + wy * (BM / 2) + mi * 16 + Mfma<T>::row(gq, q);
           const int64_t col = tn * BN + wx * (BN / 2) + ni * 16 + l16;
           T v = alpha * acc[mi][ni][q];
           if (C0) v += beta * C0[row * g.ldc0 + col];

@@ -10,6 +10,8 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   typedef v4d acc_t;
+  // row (within the 16x16 block) of accumulator element q held by lane group g
+  __host__ __device__ static constexpr int row(int g, int q) { return g + 4 * q; }
   __device__ static __forceinline__ acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -17,6 +19,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   typedef v4f acc_t;
+  __host__ __device__ static constexpr int row(int g, int q) { return 4 * g + q; }
   __device__ static __forceinline__ acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }

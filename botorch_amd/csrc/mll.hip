@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void mll_trace_kernel(TraceArgs a) {
       for (int mi = 0; mi < C::MI; ++mi)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int rl = wy * (B / 2) + mi * 16 + gq * 4 + q;
+          const int rl = wy * (B / 2) + mi * 16 + Mfma<double>::row(gq, q);
           const int gi = tm * B + rl;
           if (gi >= a.n || gi < gj) continue;
           const double wij = a.alpha[gi] * aj - acc[mi][ni][q];
