@@ -42,6 +42,7 @@ GEMM_C_LOWER = 16
 GEMM_B_KN = 32
 GEMM_F32 = 64
 GEMM_A_KM = 128
+GEMM_ACC64 = 256
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -63,8 +64,8 @@ class GpModel(ctypes.Structure):
 class QLogNEIState(ctypes.Structure):
     """bta_qlognei_state"""
     _fields_ = [
-        ("precision", _i32), ("q", _i32), ("n_b", _i32), ("S", _i32), ("ext_pad", _i64),
-        ("xb_s", _p), ("linv_ext", _p), ("Lb", _p), ("Zt", _p), ("best_f", _p),
+        ("precision", _i32), ("q", _i32), ("n_b", _i32), ("S", _i32),
+        ("xb_s", _p), ("linv", _p), ("ext", _p), ("Lb", _p), ("Zt", _p), ("best_f", _p),
         ("tau_relu", _f64), ("tau_max", _f64), ("max_tries", _i32), ("jitter", _f64),
     ]
 
@@ -94,7 +95,7 @@ SIGNATURES = {
                                         _p, _sz, _p]),
     "bta_qlognei_prepare_workspace_size": (_sz, [ctypes.POINTER(GpModel), _i32, _i32]),
     "bta_qlognei_prepare": (ctypes.c_int, [ctypes.POINTER(GpModel),
-                                           ctypes.POINTER(QLogNEIState), _p, _p, _p, _p,
+                                           ctypes.POINTER(QLogNEIState), _p, _p, _p, _p, _p,
                                            ctypes.POINTER(_f64), _p, _sz, _p]),
     "bta_qlognei_workspace_size": (_sz, [ctypes.POINTER(GpModel), ctypes.POINTER(QLogNEIState),
                                          _i64]),

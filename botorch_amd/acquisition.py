@@ -120,15 +120,17 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         Xb = X_baseline.to(device=dev, dtype=torch.float64).contiguous()
         n_b, d = Xb.shape
         S = int(self.sampler.sample_shape.numel())
-        ext_pad = _lib.round_up(n_b + 1, 128)
         n_pad = cache["n_pad"]
         st = _lib.stream()
         xb_s = torch.empty(n_b, d, dtype=torch.float64, device=dev)
         _lib.check(L.bta_scale_inputs(_lib.ptr(Xb), n_b, d, _lib.ptr(cache["center"]),
                                       _lib.ptr(cache["lengthscale"]), None, _lib.ptr(xb_s), n_b,
                                       st), "scale_inputs")
-        edtype = torch.float32 if self.precision == "mixed" else torch.float64
-        linv_ext = torch.empty(n_pad + ext_pad, n_pad, dtype=edtype, device=dev)
+        if self.precision == "mixed":
+            linv = torch.empty(n_pad, n_pad, dtype=torch.float32, device=dev)
+        else:
+            linv = cache["Linv"]
+        ext = torch.empty(n_pad, n_b + 1, dtype=torch.float64, device=dev)
         Lb = torch.empty(n_b, n_b, dtype=torch.float64, device=dev)
         best_f = torch.empty(S, dtype=torch.float64, device=dev)
         seed = self.sampler.seed
@@ -138,9 +140,9 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         state.q = 1
         state.n_b = n_b
         state.S = S
-        state.ext_pad = ext_pad
         state.xb_s = _lib.ptr(xb_s)
-        state.linv_ext = _lib.ptr(linv_ext)
+        state.linv = _lib.ptr(linv)
+        state.ext = _lib.ptr(ext)
         state.Lb = _lib.ptr(Lb)
         state.best_f = _lib.ptr(best_f)
         state.tau_relu = self.tau_relu
@@ -150,14 +152,15 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         ws = _lib.workspace(L.bta_qlognei_prepare_workspace_size(ctypes.byref(m), n_b, S), dev)
         jit = ctypes.c_double(0.0)
         _lib.check(L.bta_qlognei_prepare(ctypes.byref(m), ctypes.byref(state), _lib.ptr(Zb),
-                                         _lib.ptr(linv_ext), _lib.ptr(Lb), _lib.ptr(best_f),
+                                         _lib.ptr(linv) if self.precision == "mixed" else None,
+                                         _lib.ptr(ext), _lib.ptr(Lb), _lib.ptr(best_f),
                                          ctypes.byref(jit), _lib.ptr(ws), ws.numel(), st),
                    "qlognei_prepare")
         self.X_baseline = Xb
         self._n_b = n_b
         self._S = S
         self._state = state
-        self._keep = {"xb_s": xb_s, "linv_ext": linv_ext, "Lb": Lb, "best_f": best_f,
+        self._keep = {"xb_s": xb_s, "linv": linv, "ext": ext, "Lb": Lb, "best_f": best_f,
                       "Zb": Zb, "cache": cache}
         self._Zt = {}
         self._ws = None

@@ -15,6 +15,7 @@
 namespace bta {
 
 constexpr int MAXD = 32;
+constexpr int EXT_MAX = 257;  // 1 + n_b with n_b <= 256
 
 __global__ __launch_bounds__(256) void col_mean_kernel(const double* X, int64_t n, int d,
                                                        double* mean) {
@@ -88,13 +89,19 @@ template <typename TG, int RB, int MD>
 __global__ __launch_bounds__(256) void covar_grad_kernel(
     const double* __restrict__ X1s, int64_t n1, const double* __restrict__ X2s, int64_t n2,
     int d, int kind, double s, const TG* __restrict__ G, int64_t ldg,
-    const double* __restrict__ extra, const double* __restrict__ ls, double* __restrict__ dX1) {
+    const double* __restrict__ extra, const double* __restrict__ ls, double* __restrict__ dX1,
+    const double* __restrict__ dext, const double* __restrict__ E, int ne) {
   __shared__ double xr[RB * MD];
   __shared__ double red[4][RB * MD];
+  __shared__ double de[RB * EXT_MAX];
   const int64_t r0 = (int64_t)blockIdx.x * RB;
   for (int i = threadIdx.x; i < RB * d; i += 256) {
     int64_t r = r0 + i / d;
     xr[i] = r < n1 ? X1s[r * d + i % d] : 0.0;
+  }
+  for (int i = threadIdx.x; i < RB * ne; i += 256) {
+    int64_t r = r0 + i / ne;
+    de[i] = r < n1 ? dext[r * ne + i % ne] : 0.0;
   }
   __syncthreads();
   double acc[RB][MD];
@@ -118,7 +125,9 @@ __global__ __launch_bounds__(256) void covar_grad_kernel(
           r2 = fma(df, df, r2);
         }
       }
-      const double w = -(double)G[r * ldg + c] * kern_grad_factor(kind, r2, s, nullptr);
+      double gv = (double)G[r * ldg + c];
+      for (int j = 0; j < ne; ++j) gv = fma(de[a * ne + j], E[c * ne + j], gv);
+      const double w = -gv * kern_grad_factor(kind, r2, s, nullptr);
 #pragma unroll
       for (int j = 0; j < MD; ++j)
         if (j < d) acc[a][j] = fma(w, xr[a * d + j] - t[j], acc[a][j]);
@@ -144,6 +153,82 @@ __global__ __launch_bounds__(256) void covar_grad_kernel(
     if (extra) v += extra[r * d + j];
     dX1[r * d + j] = v / ls[j];
   }
+}
+
+
+// P[r][j0 + jj] = sum_c k(X1s[r], X2s[c]) E[c][j0 + jj]  (FP64): the products of
+// the cross-covariance rows with the few dense columns [alpha | K^-1 k(X, X_b)].
+template <int RB, int NEC>
+__global__ __launch_bounds__(256) void covar_ext_kernel(const double* __restrict__ X1s,
+                                                        int64_t n1, const double* __restrict__ X2s,
+                                                        int64_t n2, int d, int kind, double s,
+                                                        const double* __restrict__ E, int ne,
+                                                        double* __restrict__ P) {
+  __shared__ double xr[RB * MAXD];
+  __shared__ double red[4][RB * NEC];
+  const int64_t r0 = (int64_t)blockIdx.x * RB;
+  const int j0 = blockIdx.y * NEC;
+  for (int i = threadIdx.x; i < RB * d; i += 256) {
+    int64_t r = r0 + i / d;
+    xr[i] = r < n1 ? X1s[r * d + i % d] : 0.0;
+  }
+  __syncthreads();
+  double acc[RB][NEC];
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int j = 0; j < NEC; ++j) acc[a][j] = 0.0;
+  for (int64_t c = threadIdx.x; c < n2; c += 256) {
+    double t[MAXD];
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) t[j] = j < d ? X2s[c * d + j] : 0.0;
+    double e[NEC];
+#pragma unroll
+    for (int j = 0; j < NEC; ++j) e[j] = j0 + j < ne ? E[c * ne + j0 + j] : 0.0;
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+      double r2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) {
+        if (j < d) {
+          double df = xr[a * d + j] - t[j];
+          r2 = fma(df, df, r2);
+        }
+      }
+      const double kv = kern_value(kind, r2, s);
+#pragma unroll
+      for (int j = 0; j < NEC; ++j) acc[a][j] = fma(kv, e[j], acc[a][j]);
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int j = 0; j < NEC; ++j) {
+      double v = wave_sum(acc[a][j]);
+      if (lane == 0) red[wv][a * NEC + j] = v;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < RB * NEC; i += 256) {
+    const int a = i / NEC, j = i % NEC;
+    const int64_t r = r0 + a;
+    if (r < n1 && j0 + j < ne)
+      P[r * ne + j0 + j] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
+  }
+}
+
+int covar_ext(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2, int d,
+              double s, const double* E, int ne, double* P, hipStream_t st) {
+  if (d < 1 || d > MAXD || n1 < 1 || n2 < 1 || ne < 1) {
+    set_error("covar_ext: bad sizes");
+    return BTA_ERR_INVALID;
+  }
+  constexpr int RB = 4, NEC = 4;
+  dim3 grid((unsigned)((n1 + RB - 1) / RB), (unsigned)((ne + NEC - 1) / NEC));
+  hipLaunchKernelGGL((covar_ext_kernel<RB, NEC>), grid, dim3(256), 0, st, X1s, n1, X2s, n2, d,
+                     kind, s, E, ne, P);
+  BTA_LAUNCH_CHECK("covar_ext_kernel");
+  return BTA_OK;
 }
 
 int scale_inputs(const double* X, int64_t n, int d, const double* center, const double* ls,
@@ -202,34 +287,41 @@ int covar(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2
 template <int RB, int MD>
 int covar_grad_launch(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2,
                       int d, double s, const void* G, bool g_f32, int64_t ldg,
-                      const double* extra, const double* ls, double* dX1, hipStream_t st);
+                      const double* extra, const double* ls, double* dX1, hipStream_t st,
+                      const double* dext, const double* E, int ne);
 
 int covar_grad(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2, int d,
                double s, const void* G, bool g_f32, int64_t ldg, const double* extra,
-               const double* ls, double* dX1, hipStream_t st) {
-  if (d < 1 || d > MAXD || n1 < 0 || n2 < 0 || ldg < n2) {
+               const double* ls, double* dX1, hipStream_t st, const double* dext,
+               const double* E, int ne) {
+  if (d < 1 || d > MAXD || n1 < 0 || n2 < 0 || ldg < n2 || ne < 0 || ne > EXT_MAX ||
+      (ne > 0 && (!dext || !E))) {
     set_error("covar_grad: bad sizes");
     return BTA_ERR_INVALID;
   }
   if (n1 == 0) return BTA_OK;
   if (d <= 8)
-    return covar_grad_launch<4, 8>(kind, X1s, n1, X2s, n2, d, s, G, g_f32, ldg, extra, ls, dX1, st);
+    return covar_grad_launch<4, 8>(kind, X1s, n1, X2s, n2, d, s, G, g_f32, ldg, extra, ls,
+                                     dX1, st, dext, E, ne);
   if (d <= 16)
-    return covar_grad_launch<2, 16>(kind, X1s, n1, X2s, n2, d, s, G, g_f32, ldg, extra, ls, dX1, st);
-  return covar_grad_launch<1, 32>(kind, X1s, n1, X2s, n2, d, s, G, g_f32, ldg, extra, ls, dX1, st);
+    return covar_grad_launch<2, 16>(kind, X1s, n1, X2s, n2, d, s, G, g_f32, ldg, extra, ls,
+                                     dX1, st, dext, E, ne);
+  return covar_grad_launch<1, 32>(kind, X1s, n1, X2s, n2, d, s, G, g_f32, ldg, extra, ls,
+                                     dX1, st, dext, E, ne);
 }
 
 template <int RB, int MD>
 int covar_grad_launch(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2,
                       int d, double s, const void* G, bool g_f32, int64_t ldg,
-                      const double* extra, const double* ls, double* dX1, hipStream_t st) {
+                      const double* extra, const double* ls, double* dX1, hipStream_t st,
+                      const double* dext, const double* E, int ne) {
   dim3 grid((unsigned)((n1 + RB - 1) / RB));
   if (g_f32)
     hipLaunchKernelGGL((covar_grad_kernel<float, RB, MD>), grid, dim3(256), 0, st, X1s, n1, X2s,
-                       n2, d, kind, s, (const float*)G, ldg, extra, ls, dX1);
+                       n2, d, kind, s, (const float*)G, ldg, extra, ls, dX1, dext, E, ne);
   else
     hipLaunchKernelGGL((covar_grad_kernel<double, RB, MD>), grid, dim3(256), 0, st, X1s, n1,
-                       X2s, n2, d, kind, s, (const double*)G, ldg, extra, ls, dX1);
+                       X2s, n2, d, kind, s, (const double*)G, ldg, extra, ls, dX1, dext, E, ne);
   BTA_LAUNCH_CHECK("covar_grad_kernel");
   return BTA_OK;
 }
@@ -269,5 +361,5 @@ extern "C" int bta_covar_grad(int32_t kind, const double* X1s, int64_t n1, const
     return BTA_ERR_INVALID;
   }
   return bta::covar_grad(kind, X1s, n1, X2s, n2, d, outputscale, G, g_f32 != 0, ldg, extra,
-                         lengthscale, dX1, (hipStream_t)stream);
+                         lengthscale, dX1, (hipStream_t)stream, nullptr, nullptr, 0);
 }

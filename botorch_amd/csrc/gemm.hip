@@ -23,11 +23,17 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace bta {
 
-template <typename T, int BM, int BN, int BK, bool BKN, bool AKM>
+// ACC64 (float only): each K-segment of ACC_SEG columns is accumulated by the
+// FP32 MFMA and then added into FP64 accumulators, so the rounding error of a
+// dot product grows with the segment length instead of the full K.
+constexpr int ACC_SEG = 256;
+
+template <typename T, int BM, int BN, int BK, bool BKN, bool AKM, bool ACC64>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   typedef Cfg<T, BM, BN, BK, BKN, AKM> C;
   __shared__ __attribute__((aligned(16))) T lds[2 * C::STAGE];
@@ -39,7 +45,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   const int MT = g.M / BM, NT = g.N / BN;
   const int total = (g.flags & BTA_GEMM_C_LOWER) ? MT * (MT + 1) / 2 : MT * NT;
   const int W = gridDim.x, w = blockIdx.x;
-  const T alpha = (T)g.alpha, beta = (T)g.beta;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
 
@@ -56,7 +61,28 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     kb = kb / BK * BK;
     ke = (ke + BK - 1) / BK * BK;
     typename Mfma<T>::acc_t acc[C::MI][C::NI];
-    gemm_tile<T, BM, BN, BK, BKN, AKM>(A, g.lda, B, g.ldb, tm * BM, tn * BN, kb, ke, lds, acc);
+    double acc64[ACC64 ? C::MI : 1][ACC64 ? C::NI : 1][4];
+    if (ACC64) {
+#pragma unroll
+      for (int mi = 0; mi < (ACC64 ? C::MI : 1); ++mi)
+#pragma unroll
+        for (int ni = 0; ni < (ACC64 ? C::NI : 1); ++ni)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc64[mi][ni][q] = 0.0;
+      for (int k0 = kb; k0 < ke; k0 += ACC_SEG) {
+        const int k1 = min(ke, k0 + ACC_SEG);
+        gemm_tile<T, BM, BN, BK, BKN, AKM>(A, g.lda, B, g.ldb, tm * BM, tn * BN, k0, k1, lds,
+                                           acc);
+#pragma unroll
+        for (int mi = 0; mi < (ACC64 ? C::MI : 1); ++mi)
+#pragma unroll
+          for (int ni = 0; ni < (ACC64 ? C::NI : 1); ++ni)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc64[mi][ni][q] += (double)acc[mi][ni][q];
+      }
+    } else {
+      gemm_tile<T, BM, BN, BK, BKN, AKM>(A, g.lda, B, g.ldb, tm * BM, tn * BN, kb, ke, lds, acc);
+    }
 #pragma unroll
     for (int mi = 0; mi < C::MI; ++mi)
 #pragma unroll
@@ -65,9 +91,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         for (int q = 0; q < 4; ++q) {
           const int64_t row = tm * BM + wy * (BM / 2) + mi * 16 + Mfma<T>::row(gq, q);
           const int64_t col = tn * BN + wx * (BN / 2) + ni * 16 + l16;
-          T v = alpha * acc[mi][ni][q];
-          if (C0) v += beta * C0[row * g.ldc0 + col];
-          Cp[row * g.ldc + col] = v;
+          double v = ACC64 ? acc64[ACC64 ? mi : 0][ACC64 ? ni : 0][q] : (double)acc[mi][ni][q];
+          v *= g.alpha;
+          if (C0) v += g.beta * (double)C0[row * g.ldc0 + col];
+          Cp[row * g.ldc + col] = (T)v;
         }
   }
 }
@@ -162,20 +189,31 @@ static GemmPlan plan_gemm(const GemmArgs& g, int batch) {
   return p;
 }
 
-template <typename T, int BM, int BN, int BK>
-static int launch_cfg(const GemmArgs& g, int batch, int W, hipStream_t s) {
+template <typename T, int BM, int BN, int BK, bool ACC64>
+static int launch_cfg2(const GemmArgs& g, int batch, int W, hipStream_t s) {
   dim3 grid(W, batch);
   const bool bkn = g.flags & BTA_GEMM_B_KN, akm = g.flags & BTA_GEMM_A_KM;
   if (bkn && akm)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, true, ACC64>), grid, dim3(256), 0, s, g);
   else if (bkn)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, true, false, ACC64>), grid, dim3(256), 0, s,
+                       g);
   else if (akm)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, true, ACC64>), grid, dim3(256), 0, s,
+                       g);
   else
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, false, false, ACC64>), grid, dim3(256), 0, s,
+                       g);
   BTA_LAUNCH_CHECK("gemm_kernel");
   return BTA_OK;
+}
+
+template <typename T, int BM, int BN, int BK>
+static int launch_cfg(const GemmArgs& g, int batch, int W, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (g.flags & BTA_GEMM_ACC64) return launch_cfg2<T, BM, BN, BK, true>(g, batch, W, s);
+  }
+  return launch_cfg2<T, BM, BN, BK, false>(g, batch, W, s);
 }
 
 int launch_gemm(const GemmArgs& g, int batch, hipStream_t s) {

@@ -49,6 +49,14 @@ __global__ void posterior_out_kernel(const double* KA, int64_t lda, const double
   }
 }
 
+__global__ void build_ext_kernel(const double* alpha, const double* at, int64_t np, int n_b,
+                                 double* E) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= np) return;
+  E[c * (n_b + 1)] = alpha[c];
+  for (int kb = 0; kb < n_b; ++kb) E[c * (n_b + 1) + 1 + kb] = at[kb * np + c];
+}
+
 __global__ void residual_kernel(const double* y, int64_t n, double c, double* r, int64_t n_pad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_pad) r[i] = i < n ? y[i] - c : 0.0;
@@ -217,10 +225,11 @@ struct PrepWs {
 };
 
 int qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st, const double* Zb,
-                    void* linv_ext, double* Lb, double* best_f, double* jitter_used, void* ws,
+                    void* linv_out, double* ext, double* Lb, double* best_f, double* jitter_used,
+                    void* ws,
                     size_t ws_bytes, hipStream_t s) {
-  if (!m || !st || !Zb || !linv_ext || !Lb || !best_f || !m->linv || !m->alpha || !st->xb_s ||
-      st->n_b < 1 || st->S < 1 || st->ext_pad < st->n_b + 1 || st->ext_pad % 128) {
+  if (!m || !st || !Zb || !ext || !Lb || !best_f || !m->linv || !m->alpha || !st->xb_s ||
+      st->n_b < 1 || st->S < 1 || (st->precision == BTA_PREC_MIXED && !linv_out)) {
     set_error("qlognei_prepare: invalid arguments");
     return BTA_ERR_INVALID;
   }
@@ -273,21 +282,15 @@ int qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st, const do
   rc = launch_baseline_best_f(mu, m->y_mean, m->y_std, m->mean_constant, Lb, Zb, n_b, st->S,
                               best_f, s);
   if (rc) return rc;
-  // [L^-1; alpha^T; A^T; 0] in the state's precision
-  const int64_t rows = np + st->ext_pad;
-  if (st->precision == BTA_PREC_MIXED) {
-    float* E = (float*)linv_ext;
-    rc = cast_pad<float>(m->linv, np, np, np, E, np, np, np, s);
-    if (!rc) rc = cast_pad<float>(m->alpha, np, 1, np, E + np * np, np, 1, np, s);
-    if (!rc) rc = cast_pad<float>(at, np, n_b, np, E + (np + 1) * np, np, st->ext_pad - 1, np, s);
-  } else {
-    double* E = (double*)linv_ext;
-    rc = cast_pad<double>(m->linv, np, np, np, E, np, np, np, s);
-    if (!rc) rc = cast_pad<double>(m->alpha, np, 1, np, E + np * np, np, 1, np, s);
-    if (!rc)
-      rc = cast_pad<double>(at, np, n_b, np, E + (np + 1) * np, np, st->ext_pad - 1, np, s);
-  }
-  (void)rows;
+  // E[c] = [alpha_c | A[c][0..n_b)]  (FP64, n_pad x (1 + n_b))
+  hipLaunchKernelGGL(build_ext_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
+                     m->alpha, at, np, n_b, ext);
+  BTA_LAUNCH_CHECK("build_ext_kernel");
+  // L^-1 in the state's precision
+  if (st->precision == BTA_PREC_MIXED)
+    rc = cast_pad<float>(m->linv, np, np, np, (float*)linv_out, np, np, np, s);
+  else if (linv_out && linv_out != (const void*)m->linv)
+    rc = cast_pad<double>(m->linv, np, np, np, (double*)linv_out, np, np, np, s);
   return rc;
 }
 
@@ -323,9 +326,10 @@ extern "C" size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int3
 }
 
 extern "C" int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
-                                   const double* Zb, void* linv_ext, double* Lb, double* best_f,
+                                   const double* Zb, void* linv_out, double* ext, double* Lb,
+                                   double* best_f,
                                    double* jitter_used, void* ws, size_t ws_bytes,
                                    bta_stream_t stream) {
-  return bta::qlognei_prepare(m, st, Zb, linv_ext, Lb, best_f, jitter_used, ws, ws_bytes,
+  return bta::qlognei_prepare(m, st, Zb, linv_out, ext, Lb, best_f, jitter_used, ws, ws_bytes,
                               (hipStream_t)stream);
 }

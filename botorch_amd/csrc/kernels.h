@@ -12,7 +12,10 @@ int covar(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2
           int64_t sX1 = 0, int64_t sX2 = 0, int64_t sK = 0);
 int covar_grad(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2, int d,
                double s, const void* G, bool g_f32, int64_t ldg, const double* extra,
-               const double* ls, double* dX1, hipStream_t st);
+               const double* ls, double* dX1, hipStream_t st, const double* dext = nullptr,
+               const double* E = nullptr, int ne = 0);
+int covar_ext(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2, int d,
+              double s, const double* E, int ne, double* P, hipStream_t st);
 size_t potrf_ws_bytes(int64_t n_pad);
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
           double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s);

@@ -1,8 +1,8 @@
 // qLogNEI forward + backward, fused per restart.
 //
-// The posterior contraction V = K(X, X_train) L^-T (and, through the rows
-// appended to L^-1, K(X,X_train) alpha and K(X,X_train) K^-1 K(X_train,X_b))
-// comes from the MFMA GEMM.  Everything else for one restart b is small and
+// The posterior contraction V = K(X, X_train) L^-T comes from the MFMA GEMM
+// (FP64, or FP32 with FP64 segment sums); the 1 + n_b dense products
+// P = K(X, X_train) [alpha | K^-1 K(X_train, X_b)] are formed in FP64.  Everything else for one restart b is small and
 // done by one 256-thread workgroup in LDS (qlognei_mc_kernel):
 //
 //   Sigma_qq = s^2 (K(X,X) - V V^T)        Sigma_qb = s^2 (K(X,X_b) - K_x A)
@@ -15,10 +15,10 @@
 // followed by the closed-form adjoint of every step (softmax weights of the
 // logmeanexp, the fat-max Pareto derivative with the amax tie split, the
 // fat-softplus derivative, Z-contractions, the q x q Cholesky backward, the
-// triangular solve with Lb), producing
-//   dVt_ext[r] = [ -2 s^2 (gA V)_r | s dmu_r | -s^2 dSigma_qb[r] ]   for the
-// adjoint GEMM with [L^-1; alpha^T; A^T], and the direct input gradient from
-// K(X,X) and K(X,X_b).  References: utils/low_rank.py:84-172,
+// triangular solve with Lb), producing dV[r] = -2 s^2 (gA V)_r for the
+// adjoint GEMM with L^-1, dext[r] = [s dmu_r | -s^2 dSigma_qb[r]] for the FP64
+// products with [alpha | A], and the direct input gradient from K(X,X) and
+// K(X,X_b).  References: utils/low_rank.py:84-172,
 // acquisition/logei.py:376-391,688-715, utils/safe_math.py:146-191,298-355,
 // acquisition/monte_carlo.py:268-290.
 #include "bta_common.h"
@@ -40,6 +40,8 @@ struct McArgs {
   const double* Lb;
   const double* Zt;
   const double* best_f;
+  const double* P;    // R x ne: K_x [alpha | A] (FP64)
+  double* dext;       // R x ne: adjoint of P
   double* value;
   double* dxs_extra;
   int* status;
@@ -198,14 +200,14 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
                                     a.outputscale);
       sqq[i * Q + j] = ys2 * (kqq - vv);
     }
-    if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + (double)vt[(r0 + tid) * a.ldv + a.n_pad]);
+    if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + a.P[(r0 + tid) * (n_b + 1)]);
   }
   // ---- Sigma_qb into lnew[:, :n_b] ----------------------------------------
   for (int e = tid; e < Q * n_b; e += 256) {
     const int i = e / n_b, kb = e % n_b;
     const double kqb = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d),
                                   a.outputscale);
-    lnew[i * m + kb] = ys2 * (kqb - (double)vt[(r0 + i) * a.ldv + a.n_pad + 1 + kb]);
+    lnew[i * m + kb] = ys2 * (kqb - a.P[(r0 + i) * (n_b + 1) + 1 + kb]);
   }
   __syncthreads();
 
@@ -443,13 +445,9 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
       dvt[(r0 + i) * a.lddv + k] = (TV)(-2.0 * ys2 * s);
     }
   }
-  const int next = (int)(a.lddv - a.n_pad);
-  for (int e = tid; e < Q * next; e += 256) {
-    const int i = e / next, k = e % next;
-    double val = 0.0;
-    if (k == 0) val = ys * dmu[i];
-    else if (k <= n_b) val = -ys2 * dsqb[i * n_b + k - 1];
-    dvt[(r0 + i) * a.lddv + a.n_pad + k] = (TV)val;
+  for (int e = tid; e < Q * (n_b + 1); e += 256) {
+    const int i = e / (n_b + 1), k = e % (n_b + 1);
+    a.dext[(r0 + i) * (n_b + 1) + k] = k == 0 ? ys * dmu[i] : -ys2 * dsqb[i * n_b + k - 1];
   }
   // ---- direct input gradient through K(X, X) and K(X, X_b) (scaled) ------
   for (int e = tid; e < Q * d; e += 256) {
@@ -508,25 +506,23 @@ static int launch_mc(const McArgs& a, int q, int64_t b, bool f32, hipStream_t s)
 // Workspace plan for one value+grad call.
 // ---------------------------------------------------------------------------
 struct QlWs {
-  int64_t R, R_pad, ldv;
-  size_t xs, kx, vt, dvt, dxe, total;
+  int64_t R, R_pad;
+  int ne;
+  size_t xs, kx, vt, dvt, dxe, P, dext, total;
   QlWs(const bta_gp_model* m, const bta_qlognei_state* st, int64_t b) {
     const size_t es = st->precision == BTA_PREC_MIXED ? 4 : 8;
     R = b * st->q;
     R_pad = round_up(R > 0 ? R : 1, 128);
-    ldv = m->n_pad + st->ext_pad;
-    size_t o = 0;
-    auto take = [&](size_t bytes) {
-      size_t r = o;
-      o += round_up((int64_t)bytes, 256);
-      return r;
-    };
-    xs = take((size_t)R_pad * m->d * 8);
-    kx = take((size_t)R_pad * m->n_pad * es);
-    vt = take((size_t)R_pad * ldv * es);
-    dvt = take((size_t)R_pad * ldv * es);
-    dxe = take((size_t)R_pad * m->d * 8);
-    total = o;
+    ne = st->n_b + 1;
+    WsPlan p;
+    xs = p.take((size_t)R_pad * m->d * 8);
+    kx = p.take((size_t)R_pad * m->n_pad * es);
+    vt = p.take((size_t)R_pad * m->n_pad * es);
+    dvt = p.take((size_t)R_pad * m->n_pad * es);
+    dxe = p.take((size_t)R_pad * m->d * 8);
+    P = p.take((size_t)R_pad * ne * 8);
+    dext = p.take((size_t)R_pad * ne * 8);
+    total = p.off;
   }
 };
 
@@ -538,9 +534,9 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
     return BTA_ERR_INVALID;
   }
   if (st->n_b < 1 || st->n_b > 64 * MC_MAXNB_SLOTS || st->S < 1 || st->q < 1 || st->q > 8 ||
-      st->ext_pad < st->n_b + 1 || st->ext_pad % 128 || m->n_pad % 128 || m->d > MC_MAXD) {
-    set_error("qlognei_value_grad: unsupported state (n_b=%d S=%d q=%d ext_pad=%ld)", st->n_b,
-              st->S, st->q, (long)st->ext_pad);
+      !st->linv || !st->ext || m->n_pad % 128 || m->d > MC_MAXD) {
+    set_error("qlognei_value_grad: unsupported state (n_b=%d S=%d q=%d)", st->n_b, st->S,
+              st->q);
     return BTA_ERR_UNSUPPORTED;
   }
   const QlWs w(m, st, b);
@@ -554,32 +550,40 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   void* vt = base + w.vt;
   void* dvt = base + w.dvt;
   double* dxe = (double*)(base + w.dxe);
+  double* P = (double*)(base + w.P);
+  double* dext = (double*)(base + w.dext);
   const bool f32 = st->precision == BTA_PREC_MIXED;
-  const int fl = f32 ? BTA_GEMM_F32 : 0;
+  const int fl = f32 ? (BTA_GEMM_F32 | BTA_GEMM_ACC64) : 0;
+  const int64_t np = m->n_pad;
   int rc = scale_inputs(X, w.R, m->d, m->center, m->lengthscale, nullptr, xs, w.R_pad, s);
   if (rc) return rc;
-  rc = covar(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, 0.0, f32, kx, m->n_pad,
-             w.R_pad, m->n_pad, s);
+  rc = covar(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, 0.0, f32, kx, np,
+             w.R_pad, np, s);
   if (rc) return rc;
-  // [V | K_x alpha | K_x A] = K_x [L^-1; alpha^T; A^T]^T
-  rc = gemm(fl | BTA_GEMM_B_LOWER, w.R_pad, w.ldv, m->n_pad, 1.0, kx, m->n_pad, st->linv_ext,
-            m->n_pad, 0.0, nullptr, 0, vt, w.ldv, s);
+  // V = K_x L^-T  (L^-1 lower: the K-range of column tile j stops at its end)
+  rc = gemm(fl | BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, kx, np, st->linv, np, 0.0, nullptr, 0,
+            vt, np, s);
+  if (rc) return rc;
+  // P = K_x [alpha | A] in FP64
+  rc = covar_ext(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, st->ext, w.ne, P, s);
   if (rc) return rc;
   McArgs a;
   a.vt = vt;
-  a.ldv = w.ldv;
+  a.ldv = np;
   a.dvt = dvt;
-  a.lddv = w.ldv;
+  a.lddv = np;
   a.xs = xs;
   a.bs = st->xb_s;
   a.Lb = st->Lb;
   a.Zt = st->Zt;
   a.best_f = st->best_f;
+  a.P = P;
+  a.dext = dext;
   a.value = value;
   a.dxs_extra = dxe;
   a.status = status;
   a.d = m->d;
-  a.n_pad = (int)m->n_pad;
+  a.n_pad = (int)np;
   a.n_b = st->n_b;
   a.S = st->S;
   a.kind = m->kind;
@@ -594,12 +598,12 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   a.jitter = st->jitter;
   rc = launch_mc(a, st->q, b, f32, s);
   if (rc || !grad) return rc;
-  // dK_x = dVt_ext [L^-1; alpha^T; A^T]   (K x N layout of the same matrix)
-  rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, m->n_pad, w.ldv, 1.0, dvt, w.ldv,
-            st->linv_ext, m->n_pad, 0.0, nullptr, 0, kx, m->n_pad, s);
+  // dK_x = dV L^-1   (K x N layout of L^-1: rows k >= column n)
+  rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, dvt, np, st->linv, np,
+            0.0, nullptr, 0, kx, np, s);
   if (rc) return rc;
-  return covar_grad(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, kx, f32,
-                    m->n_pad, dxe, m->lengthscale, grad, s);
+  return covar_grad(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, kx, f32, np, dxe,
+                    m->lengthscale, grad, s, dext, st->ext, w.ne);
 }
 
 // ---------------------------------------------------------------------------

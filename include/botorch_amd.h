@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BTA_ABI_VERSION 1
+#define BTA_ABI_VERSION 2
 
 enum {
   BTA_OK = 0,
@@ -54,7 +54,8 @@ enum {
   BTA_GEMM_C_LOWER = 16,   /* only output tiles touching the lower triangle */
   BTA_GEMM_B_KN = 32,      /* B stored K x N row-major (else N x K row-major) */
   BTA_GEMM_F32 = 64,       /* A, B, C, C0 are float (else double) */
-  BTA_GEMM_A_KM = 128      /* A stored K x M row-major (else M x K row-major) */
+  BTA_GEMM_A_KM = 128,     /* A stored K x M row-major (else M x K row-major) */
+  BTA_GEMM_ACC64 = 256     /* float GEMM: add each 256-wide K segment into FP64 sums */
 };
 
 typedef void* bta_stream_t; /* hipStream_t */
@@ -190,12 +191,11 @@ int bta_gp_posterior(const bta_gp_model* m, const double* X, int64_t batch, int3
 typedef struct bta_qlognei_state {
   int32_t precision;        /* BTA_PREC_FP64 or BTA_PREC_MIXED */
   int32_t q;                /* 1..8 */
-  int32_t n_b;              /* pruned baseline points */
+  int32_t n_b;              /* pruned baseline points, 1..256 */
   int32_t S;                /* MC samples */
-  int64_t ext_pad;          /* rows appended to Linv: multiple of 128 >= 1 + n_b */
   const double* xb_s;       /* n_b x d scaled baseline inputs */
-  const void* linv_ext;     /* (n_pad + ext_pad) x n_pad: [L^-1; alpha^T; A^T; 0],
-                               A = K^-1 k(X, X_b); double (FP64) or float (MIXED) */
+  const void* linv;         /* n_pad x n_pad L^-1: double (FP64) or float (MIXED) */
+  const double* ext;        /* n_pad x (1 + n_b): [alpha | A], A = K^-1 k(X, X_b) */
   const double* Lb;         /* n_b x n_b cached baseline Cholesky (original scale) */
   const double* Zt;         /* (n_b + q) x S base samples, transposed */
   const double* best_f;     /* S per-sample best baseline objective */
@@ -206,14 +206,15 @@ typedef struct bta_qlognei_state {
 
 size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int32_t n_b, int32_t S);
 
-/* Builds linv_ext (given m->linv and m->alpha), Lb (psd_safe_cholesky of the
- * baseline posterior covariance, original scale) and best_f (per-sample max of
- * the baseline samples mu_b + Lb z_b, z_b = Zb (S x n_b)), as
- * acquisition/logei.py:393-459 (_init_baseline) does.  linv_ext must hold
- * (n_pad + ext_pad) x n_pad elements of the state's precision.  SYNCHRONISES. */
+/* Builds ext = [alpha | K^-1 k(X, X_b)] (n_pad x (1 + n_b), FP64), Lb
+ * (psd_safe_cholesky of the baseline posterior covariance, original scale),
+ * best_f (per-sample max of the baseline samples mu_b + Lb z_b, z_b = Zb (S x
+ * n_b)) as acquisition/logei.py:393-459 (_init_baseline) does, and, for
+ * MIXED, the float copy of L^-1 into linv_out (n_pad x n_pad; may be NULL for
+ * FP64, where the state uses m->linv).  SYNCHRONISES. */
 int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
-                        const double* Zb, void* linv_ext, double* Lb, double* best_f,
-                        double* jitter_used, void* ws, size_t ws_bytes,
+                        const double* Zb, void* linv_out, double* ext, double* Lb,
+                        double* best_f, double* jitter_used, void* ws, size_t ws_bytes,
                         bta_stream_t stream);
 
 size_t bta_qlognei_workspace_size(const bta_gp_model* m, const bta_qlognei_state* st,

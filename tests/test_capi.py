@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_metadata_without_gpu():
     lib = _lib.load()
-    assert lib.bta_abi_version() == 1
+    assert lib.bta_abi_version() == 2
     assert b"gfx950" in lib.bta_version()
 
 
@@ -41,7 +41,7 @@ def test_workspace_sizes_are_consistent():
     m = _lib.GpModel()
     m.n, m.n_pad, m.d = 2048, 2048, 12
     st = _lib.QLogNEIState()
-    st.q, st.n_b, st.S, st.ext_pad, st.precision = 4, 2, 512, 128, _lib.PREC_MIXED
+    st.q, st.n_b, st.S, st.precision = 4, 2, 512, _lib.PREC_MIXED
     ws = lib.bta_qlognei_workspace_size(ctypes.byref(m), ctypes.byref(st), 512)
     assert ws >= 2048 * 2048 * 4 * 3
     assert lib.bta_mll_workspace_size(1, 1) == 0

@@ -192,3 +192,20 @@ def test_chol_small_batched_matches_oracle():
                                         _lib.stream()))
     torch.testing.assert_close(Ad.cpu(), ref, rtol=1e-6, atol=1e-6)
     assert status.cpu()[4].item() & _lib.STATUS_JITTER
+
+
+def test_gemm_f32_acc64_segments():
+    _say("gemm f32 ACC64 (FP64 segment sums)")
+    M, N, K = 256, 256, 2048
+    g = torch.Generator().manual_seed(77)
+    A = torch.randn(M, K, generator=g, dtype=torch.float64).float().double()
+    B = torch.tril(torch.randn(N, K, generator=g, dtype=torch.float64)).float().double()
+    ref = A @ B.t()
+    Ad, Bd = A.float().to(DEV), B.float().to(DEV)
+    C = torch.empty(M, N, dtype=torch.float32, device=DEV)
+    for flags in (_lib.GEMM_F32 | _lib.GEMM_B_LOWER | _lib.GEMM_ACC64,):
+        _lib.check(_lib.lib().bta_gemm(flags, M, N, K, 1.0, _lib.ptr(Ad), K, _lib.ptr(Bd), K, 0.0,
+                                       None, 0, _lib.ptr(C), N, 1, 0, 0, 0, 0, _lib.stream()))
+        err = ((C.double().cpu() - ref).abs() / ref.abs().clamp_min(1.0)).max().item()
+        # FP64 sums of 256-long FP32 segments: error bounded by the segment length
+        assert err < 5e-6, err
