@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "qLogNEI value+grad evals/s (n=2048,d=12), mixed vs FP64 GPU; MLL-grad/s"
-PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6, "f16": 2500.0}
 
 
 def log(msg):
@@ -82,10 +82,11 @@ def gemm_fwd_roofline(acq, X, steps, precision):
     n_pad = cache["n_pad"]
     b, q, d = X.shape
     R_pad = _lib.round_up(b * q, 128)
-    dt = torch.float32 if precision == "mixed" else torch.float64
-    kx = torch.randn(R_pad, n_pad, dtype=dt, device="cuda")
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    kx = torch.rand(R_pad, n_pad, dtype=dt, device="cuda")
     vt = torch.empty(R_pad, n_pad, dtype=dt, device="cuda")
-    flags = _lib.GEMM_B_LOWER | (_lib.GEMM_F32 | _lib.GEMM_ACC64 if precision == "mixed" else 0)
+    flags = {"fp64": 0, "mixed_f32": _lib.GEMM_F32 | _lib.GEMM_ACC64,
+             "mixed": _lib.GEMM_F32 | _lib.GEMM_F16X3}[precision] | _lib.GEMM_B_LOWER
     linv = acq._keep["linv"]
 
     def launch():
@@ -178,12 +179,17 @@ def main():
     v_m, g_m, st_m = acq_m._launch(X, True)
     acq_m.check_status(st_m)
     log(f"mixed: {t_mixed * 1e3:.3f} ms/step")
-    # ---------------- FP64 HIP path ------------------------------------------------
+    # ---------------- FP64 and FP32-contraction HIP paths -------------------------
     acq_d, *_ = build_headline("fp64", n, d, b, q, S)
     t_fp64 = _timed(lambda: acq_d._launch(X, True), max(10, args.steps // 2), args.warmup)
     v_d, g_d, st_d = acq_d._launch(X, True)
     acq_d.check_status(st_d)
     log(f"fp64: {t_fp64 * 1e3:.3f} ms/step")
+    acq_f, *_ = build_headline("mixed_f32", n, d, b, q, S)
+    t_f32 = _timed(lambda: acq_f._launch(X, True), max(10, args.steps // 2), args.warmup)
+    v_f, g_f, st_f = acq_f._launch(X, True)
+    acq_f.check_status(st_f)
+    log(f"mixed_f32: {t_f32 * 1e3:.3f} ms/step")
     # ---------------- accuracy vs the FP64 oracle (same inputs, on device) ---------
     v_ref, g_ref = ref.value_and_grad(X)
 
@@ -191,6 +197,7 @@ def main():
         return float(((a - r).abs().max() / r.abs().max()).item())
 
     acc = {"mixed_value": rel(v_m, v_ref), "mixed_grad": rel(g_m, g_ref),
+           "mixed_f32_value": rel(v_f, v_ref), "mixed_f32_grad": rel(g_f, g_ref),
            "fp64_value": rel(v_d, v_ref), "fp64_grad": rel(g_d, g_ref)}
     log(f"accuracy {acc}")
     # ---------------- plain FP64 GPU path (oracle through PyTorch-ROCm) -----------
@@ -201,10 +208,17 @@ def main():
     # ---------------- roofline of the dominant kernel ------------------------------
     gemm_ms, gemm_flops = gemm_fwd_roofline(acq_m, X, 50, "mixed")
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS["f32"],
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["f32"], 4),
-            "traffic": None, "kernel": "gemm_kernel<float,128,128,16,0,0,1> V = K_x L^-T (ACC64)",
+    # split-FP16: each algorithmic FMA costs three FP16 MFMA FMAs
+    peak = PEAK_TFLOPS["f16"] / 3.0
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 1),
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "kernel": "gemm_x3_kernel<false> V = K_x L^-T (split-FP16, 3 products)",
+            "peak_basis": "dense FP16 MFMA 2500 TFLOP/s / 3 products",
             "avg_launch_ms": round(gemm_ms, 5), "flops_per_launch": gemm_flops}
+    gemm_ms32, _ = gemm_fwd_roofline(acq_f, X, 30, "mixed_f32")
+    roof_f32 = {"achieved": round(gemm_flops / (gemm_ms32 * 1e-3) / 1e12, 3),
+                "peak": PEAK_TFLOPS["f32"], "avg_launch_ms": round(gemm_ms32, 5),
+                "kernel": "gemm_kernel<float,...,ACC64> V = K_x L^-T"}
     step_flops = headline_flops(n, d, b, q, n_b, S)
     # ---------------- MLL gradient ----------------------------------------------
     mll = None
@@ -271,6 +285,8 @@ def main():
                    "precision": "mixed"},
         "restart_evals_per_s": b / t_mixed,
         "fp64_hip_steps_per_s": 1.0 / t_fp64,
+        "mixed_f32_hip_steps_per_s": 1.0 / t_f32,
+        "roofline_mixed_f32_gemm": roof_f32,
         "torch_fp64_gpu_steps_per_s": (1.0 / t_torch) if t_torch else None,
         "speedup_mixed_vs_fp64_gpu": (t_torch / t_mixed) if t_torch else None,
         "speedup_fp64hip_vs_fp64_gpu": (t_torch / t_fp64) if t_torch else None,

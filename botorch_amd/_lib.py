@@ -29,6 +29,7 @@ KERNEL_MATERN52 = 0
 KERNEL_RBF = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_MIXED_F32 = 2
 
 STATUS_JITTER = 1
 STATUS_NOT_PSD = 2
@@ -43,6 +44,7 @@ GEMM_B_KN = 32
 GEMM_F32 = 64
 GEMM_A_KM = 128
 GEMM_ACC64 = 256
+GEMM_F16X3 = 512
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -66,7 +68,8 @@ class QLogNEIState(ctypes.Structure):
     _fields_ = [
         ("precision", _i32), ("q", _i32), ("n_b", _i32), ("S", _i32),
         ("xb_s", _p), ("linv", _p), ("ext", _p), ("Lb", _p), ("Zt", _p), ("best_f", _p),
-        ("tau_relu", _f64), ("tau_max", _f64), ("max_tries", _i32), ("jitter", _f64),
+        ("tau_relu", _f64), ("tau_max", _f64), ("max_tries", _i32),
+        ("linv_scale", ctypes.c_float), ("jitter", _f64),
     ]
 
 
@@ -96,7 +99,8 @@ SIGNATURES = {
     "bta_qlognei_prepare_workspace_size": (_sz, [ctypes.POINTER(GpModel), _i32, _i32]),
     "bta_qlognei_prepare": (ctypes.c_int, [ctypes.POINTER(GpModel),
                                            ctypes.POINTER(QLogNEIState), _p, _p, _p, _p, _p,
-                                           ctypes.POINTER(_f64), _p, _sz, _p]),
+                                           ctypes.POINTER(_f64), ctypes.POINTER(ctypes.c_float),
+                                           _p, _sz, _p]),
     "bta_qlognei_workspace_size": (_sz, [ctypes.POINTER(GpModel), ctypes.POINTER(QLogNEIState),
                                          _i64]),
     "bta_qlognei_value_grad": (ctypes.c_int, [ctypes.POINTER(GpModel),

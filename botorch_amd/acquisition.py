@@ -28,6 +28,13 @@ from .sampling import SobolQMCNormalSampler, draw_sobol_normal_samples
 TAU_RELU = 1e-6  # acquisition/logei.py:67
 TAU_MAX = 1e-2  # acquisition/logei.py:68
 
+# "fp64": everything FP64; "mixed": the two L^-1 contractions on split-FP16 MFMA
+# with FP64 segment sums; "mixed_f32": the contractions on FP32 MFMA with FP64
+# segment sums.  Factors, kernel values, [alpha | A] products and the MC stage are
+# FP64 in every mode.
+_PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED,
+               "mixed_f32": _lib.PREC_MIXED_F32}
+
 
 def prune_inferior_points(model: SingleTaskGP, X: Tensor, objective=None,
                           posterior_transform=None, constraints=None, num_samples: int = 2048,
@@ -92,8 +99,8 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
             raise UnsupportedError("only fat=True (fat-max / log-fatplus) is implemented")
         if cache_root is False:
             raise UnsupportedError("only the cached-root path is implemented")
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
         for name, tau in (("tau_max", tau_max), ("tau_relu", tau_relu)):
             if not tau > 0:
                 raise ValueError(f"{name} is non-positive: {tau=}.")
@@ -126,7 +133,7 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         _lib.check(L.bta_scale_inputs(_lib.ptr(Xb), n_b, d, _lib.ptr(cache["center"]),
                                       _lib.ptr(cache["lengthscale"]), None, _lib.ptr(xb_s), n_b,
                                       st), "scale_inputs")
-        if self.precision == "mixed":
+        if self.precision != "fp64":
             linv = torch.empty(n_pad, n_pad, dtype=torch.float32, device=dev)
         else:
             linv = cache["Linv"]
@@ -136,7 +143,7 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         seed = self.sampler.seed
         Zb = draw_sobol_normal_samples(n_b, S, device=dev, seed=seed).contiguous()
         state = _lib.QLogNEIState()
-        state.precision = _lib.PREC_MIXED if self.precision == "mixed" else _lib.PREC_FP64
+        state.precision = _PRECISIONS[self.precision]
         state.q = 1
         state.n_b = n_b
         state.S = S
@@ -151,11 +158,14 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         state.jitter = 1e-8
         ws = _lib.workspace(L.bta_qlognei_prepare_workspace_size(ctypes.byref(m), n_b, S), dev)
         jit = ctypes.c_double(0.0)
+        lscale = ctypes.c_float(1.0)
         _lib.check(L.bta_qlognei_prepare(ctypes.byref(m), ctypes.byref(state), _lib.ptr(Zb),
-                                         _lib.ptr(linv) if self.precision == "mixed" else None,
+                                         _lib.ptr(linv) if self.precision != "fp64" else None,
                                          _lib.ptr(ext), _lib.ptr(Lb), _lib.ptr(best_f),
-                                         ctypes.byref(jit), _lib.ptr(ws), ws.numel(), st),
+                                         ctypes.byref(jit), ctypes.byref(lscale), _lib.ptr(ws),
+                                         ws.numel(), st),
                    "qlognei_prepare")
+        state.linv_scale = lscale.value
         self.X_baseline = Xb
         self._n_b = n_b
         self._S = S

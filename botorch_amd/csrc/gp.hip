@@ -49,6 +49,16 @@ __global__ void posterior_out_kernel(const double* KA, int64_t lda, const double
   }
 }
 
+__global__ __launch_bounds__(256) void max_abs_kernel(const double* x, int64_t n, double* out) {
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(x[i]));
+  __shared__ double red[4];
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
 __global__ void build_ext_kernel(const double* alpha, const double* at, int64_t np, int n_b,
                                  double* E) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,7 +215,7 @@ int gp_posterior(const bta_gp_model* m, const double* X, int64_t batch, int mq, 
 
 struct PrepWs {
   int64_t nbp;
-  size_t kb, vb, at, kbb, sg, lbp, arow, ka, mu, fact, total;
+  size_t kb, vb, at, kbb, sg, lbp, arow, ka, mu, fact, mx, total;
   PrepWs(const bta_gp_model* m, int n_b, int S) {
     nbp = round_up(n_b, 64);
     const int64_t np = m->n_pad;
@@ -220,16 +230,17 @@ struct PrepWs {
     ka = p.take((size_t)nbp * 64 * 8);
     mu = p.take((size_t)nbp * 8);
     fact = p.take(potrf_ws_bytes(nbp));
+    mx = p.take(64);
     total = p.off;
   }
 };
 
 int qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st, const double* Zb,
                     void* linv_out, double* ext, double* Lb, double* best_f, double* jitter_used,
-                    void* ws,
+                    float* linv_scale, void* ws,
                     size_t ws_bytes, hipStream_t s) {
   if (!m || !st || !Zb || !ext || !Lb || !best_f || !m->linv || !m->alpha || !st->xb_s ||
-      st->n_b < 1 || st->S < 1 || (st->precision == BTA_PREC_MIXED && !linv_out)) {
+      st->n_b < 1 || st->S < 1 || (st->precision != BTA_PREC_FP64 && !linv_out)) {
     set_error("qlognei_prepare: invalid arguments");
     return BTA_ERR_INVALID;
   }
@@ -286,12 +297,22 @@ int qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st, const do
   hipLaunchKernelGGL(build_ext_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
                      m->alpha, at, np, n_b, ext);
   BTA_LAUNCH_CHECK("build_ext_kernel");
-  // L^-1 in the state's precision
-  if (st->precision == BTA_PREC_MIXED)
+  // L^-1 in the state's precision, and the power-of-two scale for FP16 splits
+  if (st->precision != BTA_PREC_FP64)
     rc = cast_pad<float>(m->linv, np, np, np, (float*)linv_out, np, np, np, s);
   else if (linv_out && linv_out != (const void*)m->linv)
     rc = cast_pad<double>(m->linv, np, np, np, (double*)linv_out, np, np, np, s);
-  return rc;
+  if (rc) return rc;
+  if (linv_scale) {
+    double* mx = (double*)(base + w.mx);
+    hipLaunchKernelGGL(max_abs_kernel, dim3(1), dim3(256), 0, s, m->linv, np * np, mx);
+    BTA_LAUNCH_CHECK("max_abs_kernel");
+    double h = 0.0;
+    BTA_HIP(hipMemcpyAsync(&h, mx, sizeof(double), hipMemcpyDeviceToHost, s));
+    BTA_HIP(hipStreamSynchronize(s));
+    *linv_scale = (h > 0.0) ? (float)exp2(-ceil(log2(h))) : 1.0f;
+  }
+  return BTA_OK;
 }
 
 }  // namespace bta
@@ -327,9 +348,8 @@ extern "C" size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int3
 
 extern "C" int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
                                    const double* Zb, void* linv_out, double* ext, double* Lb,
-                                   double* best_f,
-                                   double* jitter_used, void* ws, size_t ws_bytes,
-                                   bta_stream_t stream) {
-  return bta::qlognei_prepare(m, st, Zb, linv_out, ext, Lb, best_f, jitter_used, ws, ws_bytes,
-                              (hipStream_t)stream);
+                                   double* best_f, double* jitter_used, float* linv_scale,
+                                   void* ws, size_t ws_bytes, bta_stream_t stream) {
+  return bta::qlognei_prepare(m, st, Zb, linv_out, ext, Lb, best_f, jitter_used, linv_scale, ws,
+                              ws_bytes, (hipStream_t)stream);
 }

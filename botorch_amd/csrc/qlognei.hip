@@ -42,6 +42,7 @@ struct McArgs {
   const double* best_f;
   const double* P;    // R x ne: K_x [alpha | A] (FP64)
   double* dext;       // R x ne: adjoint of P
+  float* dv_scale;    // R: power-of-two scale of each dV row (split-FP16 GEMM), or null
   double* value;
   double* dxs_extra;
   int* status;
@@ -103,6 +104,21 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red, double (
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NV; ++i) out[i] = red[i] + red[64 + i] + red[128 + i] + red[192 + i];
+  __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void block_max(double (&v)[NV], double* red, double (&out)[NV]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = wave_max(v[i]);
+    if (lane == 0) red[wv * 64 + i] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    out[i] = fmax(fmax(red[i], red[64 + i]), fmax(red[128 + i], red[192 + i]));
   __syncthreads();
 }
 
@@ -431,7 +447,30 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   }
   __syncthreads();
 
-  // ---- dVt_ext rows -------------------------------------------------------
+  // ---- dV rows (and their power-of-two scales for the split-FP16 GEMM) ----
+  if (a.dv_scale) {
+    double mx[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) mx[i] = 0.0;
+    for (int k = tid; k < a.n_pad; k += 256) {
+      double v[Q];
+#pragma unroll
+      for (int j = 0; j < Q; ++j) v[j] = (double)vt[(r0 + j) * a.ldv + k];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) s = fma(gA[i * Q + j], v[j], s);
+        mx[i] = fmax(mx[i], fabs(2.0 * ys2 * s));
+      }
+    }
+    double mxt[Q];
+    block_max<Q>(mx, red, mxt);
+    if (tid < Q) {
+      const double m = mxt[tid];
+      a.dv_scale[r0 + tid] = (m > 0.0 && isfinite(m)) ? (float)exp2(-ceil(log2(m))) : 1.0f;
+    }
+  }
   TV* dvt = (TV*)a.dvt;
   for (int k = tid; k < a.n_pad; k += 256) {
     double v[Q];
@@ -508,9 +547,9 @@ static int launch_mc(const McArgs& a, int q, int64_t b, bool f32, hipStream_t s)
 struct QlWs {
   int64_t R, R_pad;
   int ne;
-  size_t xs, kx, vt, dvt, dxe, P, dext, total;
+  size_t xs, kx, vt, dvt, dxe, P, dext, dvs, total;
   QlWs(const bta_gp_model* m, const bta_qlognei_state* st, int64_t b) {
-    const size_t es = st->precision == BTA_PREC_MIXED ? 4 : 8;
+    const size_t es = st->precision == BTA_PREC_FP64 ? 8 : 4;
     R = b * st->q;
     R_pad = round_up(R > 0 ? R : 1, 128);
     ne = st->n_b + 1;
@@ -522,6 +561,7 @@ struct QlWs {
     dxe = p.take((size_t)R_pad * m->d * 8);
     P = p.take((size_t)R_pad * ne * 8);
     dext = p.take((size_t)R_pad * ne * 8);
+    dvs = p.take((size_t)R_pad * 4);
     total = p.off;
   }
 };
@@ -552,8 +592,12 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   double* dxe = (double*)(base + w.dxe);
   double* P = (double*)(base + w.P);
   double* dext = (double*)(base + w.dext);
-  const bool f32 = st->precision == BTA_PREC_MIXED;
+  const bool f32 = st->precision != BTA_PREC_FP64;
+  const bool x3 = st->precision == BTA_PREC_MIXED;
   const int fl = f32 ? (BTA_GEMM_F32 | BTA_GEMM_ACC64) : 0;
+  float* dvs = (float*)(base + w.dvs);
+  // FP16 range for K_x: values lie in [0, outputscale]
+  const float kx_scale = (float)exp2(-ceil(log2(m->outputscale)));
   const int64_t np = m->n_pad;
   int rc = scale_inputs(X, w.R, m->d, m->center, m->lengthscale, nullptr, xs, w.R_pad, s);
   if (rc) return rc;
@@ -561,8 +605,12 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
              w.R_pad, np, s);
   if (rc) return rc;
   // V = K_x L^-T  (L^-1 lower: the K-range of column tile j stops at its end)
-  rc = gemm(fl | BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, kx, np, st->linv, np, 0.0, nullptr, 0,
-            vt, np, s);
+  if (x3)
+    rc = gemm_x3(BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, (const float*)kx, np, nullptr, kx_scale,
+                 (const float*)st->linv, np, st->linv_scale, (float*)vt, np, s);
+  else
+    rc = gemm(fl | BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, kx, np, st->linv, np, 0.0, nullptr, 0,
+              vt, np, s);
   if (rc) return rc;
   // P = K_x [alpha | A] in FP64
   rc = covar_ext(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, st->ext, w.ne, P, s);
@@ -579,6 +627,7 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   a.best_f = st->best_f;
   a.P = P;
   a.dext = dext;
+  a.dv_scale = x3 ? dvs : nullptr;
   a.value = value;
   a.dxs_extra = dxe;
   a.status = status;
@@ -599,8 +648,12 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   rc = launch_mc(a, st->q, b, f32, s);
   if (rc || !grad) return rc;
   // dK_x = dV L^-1   (K x N layout of L^-1: rows k >= column n)
-  rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, dvt, np, st->linv, np,
-            0.0, nullptr, 0, kx, np, s);
+  if (x3)
+    rc = gemm_x3(BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, (const float*)dvt, np,
+                 dvs, 1.0f, (const float*)st->linv, np, st->linv_scale, (float*)kx, np, s);
+  else
+    rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, dvt, np, st->linv,
+              np, 0.0, nullptr, 0, kx, np, s);
   if (rc) return rc;
   return covar_grad(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, kx, f32, np, dxe,
                     m->lengthscale, grad, s, dext, st->ext, w.ne);

@@ -40,7 +40,11 @@ enum {
 };
 
 enum { BTA_KERNEL_MATERN52 = 0, BTA_KERNEL_RBF = 1 };
-enum { BTA_PREC_FP64 = 0, BTA_PREC_MIXED = 1 };
+/* FP64: everything FP64.  MIXED: the two L^-1 contractions on split-FP16 MFMA
+ * (three FP16 products of hi/lo operand halves, FP64 segment sums).  MIXED_F32:
+ * the contractions on FP32 MFMA with FP64 segment sums.  Factors, kernel
+ * values, [alpha | A] products and the MC stage are FP64 in every mode. */
+enum { BTA_PREC_FP64 = 0, BTA_PREC_MIXED = 1, BTA_PREC_MIXED_F32 = 2 };
 
 /* per-restart status bits written by bta_qlognei_value_grad */
 enum { BTA_STATUS_JITTER = 1, BTA_STATUS_NOT_PSD = 2, BTA_STATUS_NAN = 4 };
@@ -55,7 +59,9 @@ enum {
   BTA_GEMM_B_KN = 32,      /* B stored K x N row-major (else N x K row-major) */
   BTA_GEMM_F32 = 64,       /* A, B, C, C0 are float (else double) */
   BTA_GEMM_A_KM = 128,     /* A stored K x M row-major (else M x K row-major) */
-  BTA_GEMM_ACC64 = 256     /* float GEMM: add each 256-wide K segment into FP64 sums */
+  BTA_GEMM_ACC64 = 256,    /* float GEMM: add each 256-wide K segment into FP64 sums */
+  BTA_GEMM_F16X3 = 512     /* float GEMM on split-FP16 MFMA (M%128, N%64, batch 1, no C0;
+                              operands must lie within the FP16 range) */
 };
 
 typedef void* bta_stream_t; /* hipStream_t */
@@ -194,13 +200,14 @@ typedef struct bta_qlognei_state {
   int32_t n_b;              /* pruned baseline points, 1..256 */
   int32_t S;                /* MC samples */
   const double* xb_s;       /* n_b x d scaled baseline inputs */
-  const void* linv;         /* n_pad x n_pad L^-1: double (FP64) or float (MIXED) */
+  const void* linv;         /* n_pad x n_pad L^-1: double (FP64) or float (MIXED*) */
   const double* ext;        /* n_pad x (1 + n_b): [alpha | A], A = K^-1 k(X, X_b) */
   const double* Lb;         /* n_b x n_b cached baseline Cholesky (original scale) */
   const double* Zt;         /* (n_b + q) x S base samples, transposed */
   const double* best_f;     /* S per-sample best baseline objective */
   double tau_relu, tau_max;
   int32_t max_tries;        /* psd_safe_cholesky tries for the q x q block */
+  float linv_scale;         /* power of two with max|L^-1| * linv_scale <= 1 (MIXED) */
   double jitter;            /* 1e-8 (FP64 default) */
 } bta_qlognei_state;
 
@@ -210,12 +217,13 @@ size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int32_t n_b, in
  * (psd_safe_cholesky of the baseline posterior covariance, original scale),
  * best_f (per-sample max of the baseline samples mu_b + Lb z_b, z_b = Zb (S x
  * n_b)) as acquisition/logei.py:393-459 (_init_baseline) does, and, for
- * MIXED, the float copy of L^-1 into linv_out (n_pad x n_pad; may be NULL for
- * FP64, where the state uses m->linv).  SYNCHRONISES. */
+ * MIXED*, the float copy of L^-1 into linv_out (n_pad x n_pad; may be NULL for
+ * FP64, where the state uses m->linv) and its power-of-two scale into
+ * *linv_scale (host, may be NULL).  SYNCHRONISES. */
 int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
                         const double* Zb, void* linv_out, double* ext, double* Lb,
-                        double* best_f, double* jitter_used, void* ws, size_t ws_bytes,
-                        bta_stream_t stream);
+                        double* best_f, double* jitter_used, float* linv_scale, void* ws,
+                        size_t ws_bytes, bta_stream_t stream);
 
 size_t bta_qlognei_workspace_size(const bta_gp_model* m, const bta_qlognei_state* st,
                                   int64_t b);

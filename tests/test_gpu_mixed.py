@@ -1,0 +1,78 @@
+"""Mixed-precision contraction paths on the GPU: the split-FP16 (x3) GEMM, the
+FP32 GEMM with FP64 segment sums, and qLogNEI in both mixed modes against the
+FP64 oracle (tolerance 1e-4 relative, the mixed-mode target)."""
+import pytest
+import torch
+
+import botorch_amd as B
+from botorch_amd import _lib
+from oracle import gp_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("flags,shape", [
+    (0, (128, 128, 256)),
+    (_lib.GEMM_B_LOWER, (256, 512, 512)),
+    (_lib.GEMM_B_KN | _lib.GEMM_B_UPPER, (256, 512, 512)),
+    (_lib.GEMM_B_LOWER, (2048, 2048, 2048)),
+])
+def test_gemm_f16x3_matches_fp64(flags, shape):
+    _say(f"gemm f16x3 flags={flags} shape={shape}")
+    M, N, K = shape
+    g = torch.Generator().manual_seed(M + N + K + flags)
+    A = (torch.rand(M, K, generator=g, dtype=torch.float64) - 0.3).float().double()
+    opB = (torch.rand(N, K, generator=g, dtype=torch.float64) - 0.5).float().double()
+    if flags & _lib.GEMM_B_LOWER:
+        opB = torch.tril(opB)
+    if flags & _lib.GEMM_B_UPPER:
+        opB = torch.triu(opB)
+    ref = A @ opB.t()
+    Bstore = opB.t().contiguous() if flags & _lib.GEMM_B_KN else opB.contiguous()
+    Ad, Bd = A.float().to(DEV), Bstore.float().to(DEV)
+    C = torch.empty(M, N, dtype=torch.float32, device=DEV)
+    ldb = N if flags & _lib.GEMM_B_KN else K
+    _lib.check(_lib.lib().bta_gemm(flags | _lib.GEMM_F32 | _lib.GEMM_F16X3, M, N, K, 1.0,
+                                   _lib.ptr(Ad), K, _lib.ptr(Bd), ldb, 0.0, None, 0, _lib.ptr(C),
+                                   N, 1, 0, 0, 0, 0, _lib.stream()))
+    err = ((C.double().cpu() - ref).abs().max() / ref.abs().max()).item()
+    # operands carried to 22 bits: error of the order of FP32 rounding
+    assert err < 2e-6, err
+
+
+def _acq_pair(n, d, q, S, precision, seed):
+    g = torch.Generator().manual_seed(seed)
+    X = torch.rand(n, d, generator=g, dtype=torch.float64)
+    Y = (torch.sin(3 * X).sum(-1) + 0.01 * torch.randn(n, generator=g,
+                                                       dtype=torch.float64)).unsqueeze(-1)
+    ls = 0.2 + 0.8 * torch.rand(d, generator=g, dtype=torch.float64)
+    base = B.MaternKernel(ard_num_dims=d)
+    base.lengthscale = ls
+    lik = B.GaussianLikelihood()
+    lik.noise = 1e-4
+    gp = B.SingleTaskGP(X.to(DEV), Y.to(DEV), likelihood=lik, covar_module=B.ScaleKernel(base))
+    orc = O.OracleGP(X, Y, O.MATERN52, ls, 1.0, 1e-4)
+    Xb, _ = O.prune_inferior_points(orc, X, num_samples=2048, seed=123)
+    acq = B.qLogNoisyExpectedImprovement(gp, Xb.to(DEV),
+                                         sampler=B.SobolQMCNormalSampler(torch.Size([S]), seed=0),
+                                         prune_baseline=False, precision=precision)
+    ref = O.OracleQLogNEI(orc, Xb, num_samples=S, seed=0)
+    Xq = torch.rand(128, q, d, generator=g, dtype=torch.float64)
+    return acq, ref, Xq
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed_f32"])
+def test_qlognei_mixed_modes_headline_shape(precision):
+    _say(f"qlognei n=2048 d=12 q=4 S=512 b=128 {precision}")
+    acq, ref, Xq = _acq_pair(2048, 12, 4, 512, precision, seed=0)
+    v_ref, g_ref = ref.value_and_grad(Xq)
+    v, gr = acq.value_and_grad(Xq.to(DEV))
+    ev = ((v.cpu() - v_ref).abs().max() / v_ref.abs().max()).item()
+    eg = ((gr.cpu() - g_ref).abs().max() / g_ref.abs().max()).item()
+    print(f"{precision}: value {ev:.2e} grad {eg:.2e}")
+    assert ev < 1e-4 and eg < 1e-4
