@@ -277,7 +277,7 @@ def main():
         "metric": METRIC, "value": 1.0 / t_mixed, "unit": "value+grad evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": t_mixed * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 (MFMA contraction) + f64 (factors, MC)",
+        "vs_baseline": None, "dtype": "f32 split into 2x f16 (MFMA contraction, f64 sums) + f64 (factors, MC)",
         "data": "synthetic",
         "config": {"workload": "qLogNEI value+grad, SingleTaskGP Matern-5/2 ARD n=2048 d=12, "
                                "b=512 restarts x q=4, S=512 Sobol samples",
