@@ -83,16 +83,29 @@ def gemm_fwd_roofline(acq, X, steps, precision):
     b, q, d = X.shape
     R_pad = _lib.round_up(b * q, 128)
     dt = torch.float64 if precision == "fp64" else torch.float32
-    kx = torch.rand(R_pad, n_pad, dtype=dt, device="cuda")
     vt = torch.empty(R_pad, n_pad, dtype=dt, device="cuda")
-    flags = {"fp64": 0, "mixed_f32": _lib.GEMM_F32 | _lib.GEMM_ACC64,
-             "mixed": _lib.GEMM_F32 | _lib.GEMM_F16X3}[precision] | _lib.GEMM_B_LOWER
     linv = acq._keep["linv"]
+    if precision == "mixed":
+        # K_x planes (values in [0, 1)) and V planes out, as the step issues it
+        kx = torch.rand(2, R_pad, n_pad, device="cuda").half()
+        vh = torch.empty(2, R_pad, n_pad, dtype=torch.float16, device="cuda")
+        pl = linv[0].numel() * 2   # bytes per L^-1 plane
+        lscale = acq._state.linv_scale
 
-    def launch():
-        _lib.check(L.bta_gemm(flags, R_pad, n_pad, n_pad, 1.0, _lib.ptr(kx), n_pad,
-                              _lib.ptr(linv), n_pad, 0.0, None, 0, _lib.ptr(vt), n_pad, 1,
-                              0, 0, 0, 0, _lib.stream()))
+        def launch():
+            _lib.check(L.bta_gemm_f16x3(_lib.GEMM_B_LOWER, R_pad, n_pad, n_pad, 1.0,
+                                        _lib.ptr(kx[0]), _lib.ptr(kx[1]), n_pad, None, 1.0,
+                                        _lib.ptr(linv[0]), _lib.ptr(linv[1]), n_pad, lscale,
+                                        _lib.ptr(vt), n_pad, _lib.ptr(vh[0]), _lib.ptr(vh[1]),
+                                        1.0, _lib.stream()))
+    else:
+        kx = torch.rand(R_pad, n_pad, dtype=dt, device="cuda")
+        flags = {"fp64": 0, "mixed_f32": _lib.GEMM_F32 | _lib.GEMM_ACC64}[precision]
+
+        def launch():
+            _lib.check(L.bta_gemm(flags | _lib.GEMM_B_LOWER, R_pad, n_pad, n_pad, 1.0,
+                                  _lib.ptr(kx), n_pad, _lib.ptr(linv), n_pad, 0.0, None, 0,
+                                  _lib.ptr(vt), n_pad, 1, 0, 0, 0, 0, _lib.stream()))
 
     for _ in range(3):
         launch()
@@ -212,7 +225,7 @@ def main():
     peak = PEAK_TFLOPS["f16"] / 3.0
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 1),
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-            "kernel": "gemm_x3_kernel<false> V = K_x L^-T (split-FP16, 3 products)",
+            "kernel": "gemm_h2_kernel V = K_x L^-T (split-FP16 planes, 3 products)",
             "peak_basis": "dense FP16 MFMA 2500 TFLOP/s / 3 products",
             "avg_launch_ms": round(gemm_ms, 5), "flops_per_launch": gemm_flops}
     gemm_ms32, _ = gemm_fwd_roofline(acq_f, X, 30, "mixed_f32")

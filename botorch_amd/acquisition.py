@@ -133,7 +133,9 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         _lib.check(L.bta_scale_inputs(_lib.ptr(Xb), n_b, d, _lib.ptr(cache["center"]),
                                       _lib.ptr(cache["lengthscale"]), None, _lib.ptr(xb_s), n_b,
                                       st), "scale_inputs")
-        if self.precision != "fp64":
+        if self.precision == "mixed":  # hi/lo FP16 planes of c L^-1 and of c L^-T
+            linv = torch.empty(4, n_pad, n_pad, dtype=torch.float16, device=dev)
+        elif self.precision == "mixed_f32":
             linv = torch.empty(n_pad, n_pad, dtype=torch.float32, device=dev)
         else:
             linv = cache["Linv"]

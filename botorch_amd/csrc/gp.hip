@@ -14,6 +14,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cmath>
 
 namespace bta {
 
@@ -49,14 +50,17 @@ __global__ void posterior_out_kernel(const double* KA, int64_t lda, const double
   }
 }
 
+// out[blockIdx.x] = max |x[i]| over the block's grid-stride share
+constexpr int MAXABS_BLOCKS = 256;
 __global__ __launch_bounds__(256) void max_abs_kernel(const double* x, int64_t n, double* out) {
   double m = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(x[i]));
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    m = fmax(m, fabs(x[i]));
   __shared__ double red[4];
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) *out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  if (threadIdx.x == 0) out[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
 __global__ void build_ext_kernel(const double* alpha, const double* at, int64_t np, int n_b,
@@ -230,7 +234,7 @@ struct PrepWs {
     ka = p.take((size_t)nbp * 64 * 8);
     mu = p.take((size_t)nbp * 8);
     fact = p.take(potrf_ws_bytes(nbp));
-    mx = p.take(64);
+    mx = p.take((MAXABS_BLOCKS + 1) * sizeof(double));
     total = p.off;
   }
 };
@@ -297,22 +301,37 @@ int qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st, const do
   hipLaunchKernelGGL(build_ext_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
                      m->alpha, at, np, n_b, ext);
   BTA_LAUNCH_CHECK("build_ext_kernel");
-  // L^-1 in the state's precision, and the power-of-two scale for FP16 splits
-  if (st->precision != BTA_PREC_FP64)
-    rc = cast_pad<float>(m->linv, np, np, np, (float*)linv_out, np, np, np, s);
-  else if (linv_out && linv_out != (const void*)m->linv)
-    rc = cast_pad<double>(m->linv, np, np, np, (double*)linv_out, np, np, np, s);
-  if (rc) return rc;
-  if (linv_scale) {
+  // the power-of-two scale that puts L^-1 in the FP16 range
+  float lscale = 1.0f;
+  if (linv_scale || st->precision == BTA_PREC_MIXED) {
     double* mx = (double*)(base + w.mx);
-    hipLaunchKernelGGL(max_abs_kernel, dim3(1), dim3(256), 0, s, m->linv, np * np, mx);
+    const unsigned nblk = MAXABS_BLOCKS;
+    hipLaunchKernelGGL(max_abs_kernel, dim3(nblk), dim3(256), 0, s, m->linv, np * np, mx);
+    BTA_LAUNCH_CHECK("max_abs_kernel");
+    hipLaunchKernelGGL(max_abs_kernel, dim3(1), dim3(256), 0, s, mx, (int64_t)nblk,
+                       mx + nblk);
     BTA_LAUNCH_CHECK("max_abs_kernel");
     double h = 0.0;
-    BTA_HIP(hipMemcpyAsync(&h, mx, sizeof(double), hipMemcpyDeviceToHost, s));
+    BTA_HIP(hipMemcpyAsync(&h, mx + nblk, sizeof(double), hipMemcpyDeviceToHost, s));
     BTA_HIP(hipStreamSynchronize(s));
-    *linv_scale = (h > 0.0) ? (float)exp2(-ceil(log2(h))) : 1.0f;
+    lscale = (h > 0.0 && std::isfinite(h)) ? (float)exp2(-ceil(log2(h))) : 1.0f;
+    if (linv_scale) *linv_scale = lscale;
   }
-  return BTA_OK;
+  // L^-1 in the state's precision: FP32, or the four FP16 planes
+  // [hi(c L^-1) | lo(c L^-1) | hi(c L^-T) | lo(c L^-T)] with c = lscale
+  if (st->precision == BTA_PREC_MIXED) {
+    _Float16* pl = (_Float16*)linv_out;
+    const size_t pn = (size_t)np * np;
+    rc = split_f16(true, m->linv, np, np, np, np, np, lscale, false, pl, pl + pn, np, s);
+    if (rc) return rc;
+    rc = split_f16(true, m->linv, np, np, np, np, np, lscale, true, pl + 2 * pn, pl + 3 * pn, np,
+                   s);
+  } else if (st->precision == BTA_PREC_MIXED_F32) {
+    rc = cast_pad<float>(m->linv, np, np, np, (float*)linv_out, np, np, np, s);
+  } else if (linv_out && linv_out != (const void*)m->linv) {
+    rc = cast_pad<double>(m->linv, np, np, np, (double*)linv_out, np, np, np, s);
+  }
+  return rc;
 }
 
 }  // namespace bta

@@ -33,6 +33,62 @@ int gemm_x3(int flags, int64_t M, int64_t N, int64_t K, double alpha, const floa
             int64_t lda, const float* a_row_scale, float a_scale, const float* B, int64_t ldb,
             float b_scale, float* C, int64_t ldc, hipStream_t s);
 
+// Split-FP16 GEMM on pre-split operands (gemm_h2.hip).
+struct H2Args {
+  const _Float16 *Ah, *Al;  // M x K planes (row stride lda halfs)
+  const _Float16 *Bh, *Bl;  // N x K planes (row stride ldb halfs)
+  float* C;                 // M x N (row stride ldc)
+  _Float16 *Ch, *Cl;        // optional planes of c_scale * C (row stride ldc)
+  const float* a_row_scale; // optional per-row scale of A
+  float a_scale, b_scale, c_scale;
+  int64_t lda, ldb, ldc;
+  int M, N, K, flags;       // flags: BTA_GEMM_B_LOWER / BTA_GEMM_B_UPPER (N x K view)
+  double alpha;
+};
+int gemm_h2(const H2Args& a, hipStream_t s);
+int split_f16(bool f64, const void* X, int64_t ldx, int64_t rows, int64_t cols,
+              int64_t rows_pad, int64_t cols_pad, float scale, bool transpose, _Float16* hi,
+              _Float16* lo, int64_t ldo, hipStream_t s);
+
+// Fused elementwise stages of the qLogNEI step (qstep.hip).
+constexpr int QS_NE_MAX = 16;  // columns of [alpha | A] staged per pass
+constexpr int QS_COL_TILE = 64;
+
+struct QsCovarArgs {
+  const double* X;        // R x d raw query rows
+  const double* center;   // d
+  const double* ls;       // d
+  const double* T;        // n x d scaled training inputs
+  const double* E;        // n x ne  [alpha | A]
+  void* K;                // R_pad x ldk (FP64 / FP32 / FP16 hi plane)
+  _Float16* Klo;          // FP16 lo plane (mode 2)
+  double* xs_out;         // R_pad x d scaled rows (zero beyond R)
+  double* Ppart;          // (n_pad / 64) x R x ne partial products
+  int64_t R, R_pad, n, n_pad, ldk;
+  int d, kind, ne;
+  double s;
+  float kscale;
+};
+
+// The adjoint of K_x is dK_x[r] = sum_{j<q} gq[r][j] U[q floor(r/q) + j] with
+// U = K_x K^-1 (= V L^-1): the q rows of a restart mix through its q x q block.
+struct QsGradArgs {
+  const double* xs;       // R x d scaled query rows
+  const double* T;        // n x d scaled training inputs
+  const double* E;        // n x ne
+  const void* U;          // R x ldu, FP64 or FP32
+  const double* gq;       // R x q
+  const double* dext;     // R x ne adjoint of P
+  double* Dpart;          // (n_pad / 64) x R x d partial input gradients
+  int64_t R, R_pad, n, n_pad, ldu;
+  int d, kind, ne, q;
+  double s;
+};
+
+int qstep_covar(const QsCovarArgs& a, int mode, hipStream_t st);
+int qstep_grad(const QsGradArgs& a, bool u_f32, const double* extra, const double* ls,
+               double* dX, hipStream_t st);
+
 // Bump allocator over a caller workspace (256-byte aligned slices).
 struct WsPlan {
   size_t off = 0;

@@ -33,16 +33,16 @@ constexpr int MC_MAXNB_SLOTS = 4;  // n_b <= 256
 struct McArgs {
   const void* vt;
   int64_t ldv;
-  void* dvt;
-  int64_t lddv;
+  double* gq;         // R x q: dV = gq V per restart block (-2 s^2 gA)
   const double* xs;
   const double* bs;
   const double* Lb;
   const double* Zt;
   const double* best_f;
-  const double* P;    // R x ne: K_x [alpha | A] (FP64)
+  const double* P;    // p_parts x (R x ne) partial sums of K_x [alpha | A] (FP64)
+  int64_t p_stride;   // elements between partials
+  int p_parts;
   double* dext;       // R x ne: adjoint of P
-  float* dv_scale;    // R: power-of-two scale of each dV row (split-FP16 GEMM), or null
   double* value;
   double* dxs_extra;
   int* status;
@@ -72,6 +72,14 @@ struct McLayout {
     total = o;
   }
 };
+
+// P[r][j] as the fixed-order sum of its column-tile partials
+__device__ __forceinline__ double mc_p(const McArgs& a, int64_t r, int j) {
+  const double* p = a.P + r * (a.n_b + 1) + j;
+  double s = 0.0;
+  for (int t = 0; t < a.p_parts; ++t) s += p[t * a.p_stride];
+  return s;
+}
 
 __device__ __forceinline__ void log_fatplus_grad(double x, double tau, double* v, double* dv) {
   // log(tau * (softplus(x/tau) + 0.1 / (1 + (x/tau)^2)))  utils/safe_math.py:298-325
@@ -216,14 +224,14 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
                                     a.outputscale);
       sqq[i * Q + j] = ys2 * (kqq - vv);
     }
-    if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + a.P[(r0 + tid) * (n_b + 1)]);
+    if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + mc_p(a, r0 + tid, 0));
   }
   // ---- Sigma_qb into lnew[:, :n_b] ----------------------------------------
   for (int e = tid; e < Q * n_b; e += 256) {
     const int i = e / n_b, kb = e % n_b;
     const double kqb = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d),
                                   a.outputscale);
-    lnew[i * m + kb] = ys2 * (kqb - a.P[(r0 + i) * (n_b + 1) + 1 + kb]);
+    lnew[i * m + kb] = ys2 * (kqb - mc_p(a, r0 + i, 1 + kb));
   }
   __syncthreads();
 
@@ -447,43 +455,8 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   }
   __syncthreads();
 
-  // ---- dV rows (and their power-of-two scales for the split-FP16 GEMM) ----
-  if (a.dv_scale) {
-    double mx[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) mx[i] = 0.0;
-    for (int k = tid; k < a.n_pad; k += 256) {
-      double v[Q];
-#pragma unroll
-      for (int j = 0; j < Q; ++j) v[j] = (double)vt[(r0 + j) * a.ldv + k];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < Q; ++j) s = fma(gA[i * Q + j], v[j], s);
-        mx[i] = fmax(mx[i], fabs(2.0 * ys2 * s));
-      }
-    }
-    double mxt[Q];
-    block_max<Q>(mx, red, mxt);
-    if (tid < Q) {
-      const double m = mxt[tid];
-      a.dv_scale[r0 + tid] = (m > 0.0 && isfinite(m)) ? (float)exp2(-ceil(log2(m))) : 1.0f;
-    }
-  }
-  TV* dvt = (TV*)a.dvt;
-  for (int k = tid; k < a.n_pad; k += 256) {
-    double v[Q];
-#pragma unroll
-    for (int j = 0; j < Q; ++j) v[j] = (double)vt[(r0 + j) * a.ldv + k];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      double s = 0.0;
-#pragma unroll
-      for (int j = 0; j < Q; ++j) s = fma(gA[i * Q + j], v[j], s);
-      dvt[(r0 + i) * a.lddv + k] = (TV)(-2.0 * ys2 * s);
-    }
-  }
+  // ---- dV = gq V with gq = -2 s^2 gA: the q x q block for the gradient stage
+  for (int e = tid; e < Q * Q; e += 256) a.gq[r0 * Q + e] = -2.0 * ys2 * gA[e];
   for (int e = tid; e < Q * (n_b + 1); e += 256) {
     const int i = e / (n_b + 1), k = e % (n_b + 1);
     a.dext[(r0 + i) * (n_b + 1) + k] = k == 0 ? ys * dmu[i] : -ys2 * dsqb[i * n_b + k - 1];
@@ -544,24 +517,29 @@ static int launch_mc(const McArgs& a, int q, int64_t b, bool f32, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Workspace plan for one value+grad call.
 // ---------------------------------------------------------------------------
+// kx holds K_x (FP64, FP32, or its two FP16 planes) and is then reused for
+// U = V L^-1 (same element size); vh holds V's FP16 planes (split mode).
 struct QlWs {
   int64_t R, R_pad;
-  int ne;
-  size_t xs, kx, vt, dvt, dxe, P, dext, dvs, total;
+  int ne, parts;  // parts: column-tile partials of P and of the input gradient
+  size_t xs, kx, vt, vh, dxe, P, dext, gq, Dpart, total;
   QlWs(const bta_gp_model* m, const bta_qlognei_state* st, int64_t b) {
     const size_t es = st->precision == BTA_PREC_FP64 ? 8 : 4;
     R = b * st->q;
     R_pad = round_up(R > 0 ? R : 1, 128);
     ne = st->n_b + 1;
+    parts = (int)(m->n_pad / QS_COL_TILE);
+    const size_t nn = (size_t)R_pad * m->n_pad;
     WsPlan p;
     xs = p.take((size_t)R_pad * m->d * 8);
-    kx = p.take((size_t)R_pad * m->n_pad * es);
-    vt = p.take((size_t)R_pad * m->n_pad * es);
-    dvt = p.take((size_t)R_pad * m->n_pad * es);
+    kx = p.take(nn * es);
+    vt = p.take(nn * es);
+    vh = p.take(st->precision == BTA_PREC_MIXED ? nn * 4 : 0);
     dxe = p.take((size_t)R_pad * m->d * 8);
-    P = p.take((size_t)R_pad * ne * 8);
+    P = p.take((size_t)parts * R_pad * ne * 8);
     dext = p.take((size_t)R_pad * ne * 8);
-    dvs = p.take((size_t)R_pad * 4);
+    gq = p.take((size_t)R_pad * st->q * 8);
+    Dpart = p.take((size_t)parts * R_pad * m->d * 8);
     total = p.off;
   }
 };
@@ -588,46 +566,118 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   double* xs = (double*)(base + w.xs);
   void* kx = base + w.kx;
   void* vt = base + w.vt;
-  void* dvt = base + w.dvt;
   double* dxe = (double*)(base + w.dxe);
   double* P = (double*)(base + w.P);
   double* dext = (double*)(base + w.dext);
+  double* gq = (double*)(base + w.gq);
   const bool f32 = st->precision != BTA_PREC_FP64;
-  const bool x3 = st->precision == BTA_PREC_MIXED;
+  const bool split = st->precision == BTA_PREC_MIXED;
   const int fl = f32 ? (BTA_GEMM_F32 | BTA_GEMM_ACC64) : 0;
-  float* dvs = (float*)(base + w.dvs);
-  // FP16 range for K_x: values lie in [0, outputscale]
-  const float kx_scale = (float)exp2(-ceil(log2(m->outputscale)));
   const int64_t np = m->n_pad;
-  int rc = scale_inputs(X, w.R, m->d, m->center, m->lengthscale, nullptr, xs, w.R_pad, s);
-  if (rc) return rc;
-  rc = covar(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, 0.0, f32, kx, np,
-             w.R_pad, np, s);
+  const size_t plane = (size_t)w.R_pad * np;  // halfs per FP16 plane of K_x / V
+  // FP16 ranges: K_x in [0, s]; |V_rj| <= sqrt(s) since V V^T <= K(X, X)
+  const float kx_scale = (float)exp2(-ceil(log2(m->outputscale)));
+  const float v_scale = (float)exp2(-ceil(0.5 * log2(m->outputscale)));
+  _Float16* kh = (_Float16*)kx;
+  _Float16* vh = (_Float16*)(base + w.vh);
+  // L^-1 planes of the split mode: [hi(L^-1) | lo(L^-1) | hi(L^-T) | lo(L^-T)]
+  const _Float16* lh = (const _Float16*)st->linv;
+  const size_t lplane = (size_t)np * np;
+
+  // xs, K_x and the column-tile partials of P = K_x [alpha | A] in one pass
+  QsCovarArgs c;
+  c.X = X;
+  c.center = m->center;
+  c.ls = m->lengthscale;
+  c.T = m->train_xs;
+  c.E = st->ext;
+  c.K = kx;
+  c.Klo = split ? kh + plane : nullptr;
+  c.xs_out = xs;
+  c.Ppart = P;
+  c.R = w.R;
+  c.R_pad = w.R_pad;
+  c.n = m->n;
+  c.n_pad = np;
+  c.ldk = np;
+  c.d = m->d;
+  c.kind = m->kind;
+  c.ne = w.ne;
+  c.s = m->outputscale;
+  c.kscale = kx_scale;
+  int rc = qstep_covar(c, split ? 2 : (f32 ? 1 : 0), s);
   if (rc) return rc;
   // V = K_x L^-T  (L^-1 lower: the K-range of column tile j stops at its end)
-  if (x3)
-    rc = gemm_x3(BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, (const float*)kx, np, nullptr, kx_scale,
-                 (const float*)st->linv, np, st->linv_scale, (float*)vt, np, s);
-  else
+  if (split) {
+    H2Args h;
+    h.Ah = kh;
+    h.Al = kh + plane;
+    h.Bh = lh;
+    h.Bl = lh + lplane;
+    h.C = (float*)vt;
+    h.Ch = grad ? vh : nullptr;
+    h.Cl = grad ? vh + plane : nullptr;
+    h.a_row_scale = nullptr;
+    h.a_scale = kx_scale;
+    h.b_scale = st->linv_scale;
+    h.c_scale = v_scale;
+    h.lda = np;
+    h.ldb = np;
+    h.ldc = np;
+    h.M = (int)w.R_pad;
+    h.N = (int)np;
+    h.K = (int)np;
+    h.flags = BTA_GEMM_B_LOWER;
+    h.alpha = 1.0;
+    rc = gemm_h2(h, s);
+  } else {
     rc = gemm(fl | BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, kx, np, st->linv, np, 0.0, nullptr, 0,
               vt, np, s);
+  }
   if (rc) return rc;
-  // P = K_x [alpha | A] in FP64
-  rc = covar_ext(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, st->ext, w.ne, P, s);
-  if (rc) return rc;
+  // U = V L^-1 = K_x K^-1 into kx (K x N view of L^-1: rows k >= column n)
+  if (grad) {
+    if (split) {
+      H2Args h;
+      h.Ah = vh;
+      h.Al = vh + plane;
+      h.Bh = lh + 2 * lplane;
+      h.Bl = lh + 3 * lplane;
+      h.C = (float*)kx;
+      h.Ch = nullptr;
+      h.Cl = nullptr;
+      h.a_row_scale = nullptr;
+      h.a_scale = v_scale;
+      h.b_scale = st->linv_scale;
+      h.c_scale = 1.0f;
+      h.lda = np;
+      h.ldb = np;
+      h.ldc = np;
+      h.M = (int)w.R_pad;
+      h.N = (int)np;
+      h.K = (int)np;
+      h.flags = BTA_GEMM_B_UPPER;
+      h.alpha = 1.0;
+      rc = gemm_h2(h, s);
+    } else {
+      rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, vt, np, st->linv,
+                np, 0.0, nullptr, 0, kx, np, s);
+    }
+    if (rc) return rc;
+  }
   McArgs a;
   a.vt = vt;
   a.ldv = np;
-  a.dvt = dvt;
-  a.lddv = np;
+  a.gq = gq;
   a.xs = xs;
   a.bs = st->xb_s;
   a.Lb = st->Lb;
   a.Zt = st->Zt;
   a.best_f = st->best_f;
   a.P = P;
+  a.p_parts = w.parts;
+  a.p_stride = w.R * w.ne;
   a.dext = dext;
-  a.dv_scale = x3 ? dvs : nullptr;
   a.value = value;
   a.dxs_extra = dxe;
   a.status = status;
@@ -647,16 +697,26 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   a.jitter = st->jitter;
   rc = launch_mc(a, st->q, b, f32, s);
   if (rc || !grad) return rc;
-  // dK_x = dV L^-1   (K x N layout of L^-1: rows k >= column n)
-  if (x3)
-    rc = gemm_x3(BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, (const float*)dvt, np,
-                 dvs, 1.0f, (const float*)st->linv, np, st->linv_scale, (float*)kx, np, s);
-  else
-    rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, dvt, np, st->linv,
-              np, 0.0, nullptr, 0, kx, np, s);
-  if (rc) return rc;
-  return covar_grad(m->kind, xs, w.R, m->train_xs, m->n, m->d, m->outputscale, kx, f32, np, dxe,
-                    m->lengthscale, grad, s, dext, st->ext, w.ne);
+  // dK_x = gq U per restart block, contracted with dK/dxs on the fly
+  QsGradArgs g;
+  g.xs = xs;
+  g.T = m->train_xs;
+  g.E = st->ext;
+  g.U = kx;
+  g.gq = gq;
+  g.dext = dext;
+  g.Dpart = (double*)(base + w.Dpart);
+  g.R = w.R;
+  g.R_pad = w.R_pad;
+  g.n = m->n;
+  g.n_pad = np;
+  g.ldu = np;
+  g.d = m->d;
+  g.kind = m->kind;
+  g.ne = w.ne;
+  g.q = st->q;
+  g.s = m->outputscale;
+  return qstep_grad(g, f32, dxe, m->lengthscale, grad, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BTA_ABI_VERSION 2
+#define BTA_ABI_VERSION 3
 
 enum {
   BTA_OK = 0,
@@ -41,7 +41,7 @@ enum {
 
 enum { BTA_KERNEL_MATERN52 = 0, BTA_KERNEL_RBF = 1 };
 /* FP64: everything FP64.  MIXED: the two L^-1 contractions on split-FP16 MFMA
- * (three FP16 products of hi/lo operand halves, FP64 segment sums).  MIXED_F32:
+ * (three FP16 products of hi/lo operand planes, FP64 segment sums).  MIXED_F32:
  * the contractions on FP32 MFMA with FP64 segment sums.  Factors, kernel
  * values, [alpha | A] products and the MC stage are FP64 in every mode. */
 enum { BTA_PREC_FP64 = 0, BTA_PREC_MIXED = 1, BTA_PREC_MIXED_F32 = 2 };
@@ -114,6 +114,29 @@ int bta_gemm(int32_t flags, int64_t M, int64_t N, int64_t K, double alpha, const
              int64_t lda, const void* B, int64_t ldb, double beta, const void* C0,
              int64_t ldc0, void* C, int64_t ldc, int64_t batch, int64_t sA, int64_t sB,
              int64_t sC0, int64_t sC, bta_stream_t stream);
+
+/* FP16 hi/lo planes of scale * X (X double when src_f64, else float; rows x
+ * cols, row stride ldx), or of scale * X^T when transpose != 0: hi = fp16(x),
+ * lo = fp16(x - hi) with x = fp32(scale * X).  Output rows_pad x cols_pad
+ * (cols_pad x rows_pad transposed), row stride ldo, zero in the padding.
+ * Operand format of bta_gemm_f16x3; no counterpart in the reference (an
+ * internal building block of the split-FP16 contractions). */
+int bta_split_f16(int32_t src_f64, const void* X, int64_t ldx, int64_t rows, int64_t cols,
+                  int64_t rows_pad, int64_t cols_pad, float scale, int32_t transpose,
+                  void* hi, void* lo, int64_t ldo, bta_stream_t stream);
+
+/* C = alpha (A_hi + A_lo)(B_hi + B_lo)^T / (a_scale * a_row_scale[m] * b_scale)
+ * without the lo*lo term, on FP16 MFMA with FP64 sums of 256-long K segments.
+ * A planes M x K (stride lda), B planes N x K (stride ldb), FP16; C float M x N
+ * (stride ldc).  When c_hi is not NULL the planes of c_scale * C are also
+ * written (stride ldc).  flags: BTA_GEMM_B_LOWER / BTA_GEMM_B_UPPER.  M, N
+ * multiples of 128, K of 32, lda/ldb of 8; a_row_scale may be NULL.  No
+ * counterpart in the reference: the contraction kernel of BTA_PREC_MIXED. */
+int bta_gemm_f16x3(int32_t flags, int64_t M, int64_t N, int64_t K, double alpha,
+                   const void* a_hi, const void* a_lo, int64_t lda, const float* a_row_scale,
+                   float a_scale, const void* b_hi, const void* b_lo, int64_t ldb,
+                   float b_scale, float* C, int64_t ldc, void* c_hi, void* c_lo,
+                   float c_scale, bta_stream_t stream);
 
 size_t bta_potrf_workspace_size(int64_t n_pad);
 
@@ -195,12 +218,15 @@ int bta_gp_posterior(const bta_gp_model* m, const double* X, int64_t batch, int3
 /* by generation/gen.py:423-485 (f_np_wrapper).                              */
 /* ------------------------------------------------------------------------ */
 typedef struct bta_qlognei_state {
-  int32_t precision;        /* BTA_PREC_FP64 or BTA_PREC_MIXED */
+  int32_t precision;        /* BTA_PREC_* */
   int32_t q;                /* 1..8 */
   int32_t n_b;              /* pruned baseline points, 1..256 */
   int32_t S;                /* MC samples */
   const double* xb_s;       /* n_b x d scaled baseline inputs */
-  const void* linv;         /* n_pad x n_pad L^-1: double (FP64) or float (MIXED*) */
+  const void* linv;         /* L^-1 as prepared: n_pad x n_pad double (FP64), float
+                               (MIXED_F32), or four n_pad x n_pad FP16 planes (MIXED):
+                               [hi(cL^-1) | lo(cL^-1) | hi(cL^-T) | lo(cL^-T)],
+                               c = linv_scale */
   const double* ext;        /* n_pad x (1 + n_b): [alpha | A], A = K^-1 k(X, X_b) */
   const double* Lb;         /* n_b x n_b cached baseline Cholesky (original scale) */
   const double* Zt;         /* (n_b + q) x S base samples, transposed */
@@ -216,10 +242,11 @@ size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int32_t n_b, in
 /* Builds ext = [alpha | K^-1 k(X, X_b)] (n_pad x (1 + n_b), FP64), Lb
  * (psd_safe_cholesky of the baseline posterior covariance, original scale),
  * best_f (per-sample max of the baseline samples mu_b + Lb z_b, z_b = Zb (S x
- * n_b)) as acquisition/logei.py:393-459 (_init_baseline) does, and, for
- * MIXED*, the float copy of L^-1 into linv_out (n_pad x n_pad; may be NULL for
- * FP64, where the state uses m->linv) and its power-of-two scale into
- * *linv_scale (host, may be NULL).  SYNCHRONISES. */
+ * n_b)) as acquisition/logei.py:393-459 (_init_baseline) does, L^-1 in the
+ * state's precision into linv_out (MIXED_F32: n_pad^2 floats; MIXED: the four
+ * FP16 planes of bta_qlognei_state.linv, 4 n_pad^2 halfs; may be NULL for FP64,
+ * where the state uses m->linv) and the power-of-two scale c with max|c L^-1|
+ * <= 1 into *linv_scale (host, may be NULL).  SYNCHRONISES. */
 int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
                         const double* Zb, void* linv_out, double* ext, double* Lb,
                         double* best_f, double* jitter_used, float* linv_scale, void* ws,

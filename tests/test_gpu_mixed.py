@@ -45,6 +45,76 @@ def test_gemm_f16x3_matches_fp64(flags, shape):
     assert err < 2e-6, err
 
 
+def _split(X, rows_pad=None, cols_pad=None, scale=1.0, transpose=False):
+    """FP16 hi/lo planes of scale * X through bta_split_f16 (X double on the device)."""
+    rows, cols = X.shape
+    rows_pad = rows if rows_pad is None else rows_pad
+    cols_pad = cols if cols_pad is None else cols_pad
+    shape = (cols_pad, rows_pad) if transpose else (rows_pad, cols_pad)
+    hi = torch.full(shape, float("nan"), dtype=torch.float16, device=DEV)
+    lo = torch.full(shape, float("nan"), dtype=torch.float16, device=DEV)
+    _lib.check(_lib.lib().bta_split_f16(1, _lib.ptr(X), cols, rows, cols, rows_pad, cols_pad,
+                                        scale, int(transpose), _lib.ptr(hi), _lib.ptr(lo),
+                                        shape[1], _lib.stream()))
+    return hi, lo
+
+
+def test_split_f16_planes():
+    _say("split_f16 planes, transpose and padding")
+    g = torch.Generator().manual_seed(5)
+    X = (torch.randn(200, 136, generator=g, dtype=torch.float64) * 10).to(DEV)
+    hi, lo = _split(X, 256, 192, scale=0.125)
+    x32 = (X * 0.125).float()
+    # hi = fp16(x), lo = fp16(x - hi): the pair carries x to 2^-22 relative
+    assert torch.equal(hi[:200, :136], x32.half())
+    assert torch.equal(lo[:200, :136], (x32 - x32.half().float()).half())
+    rec = hi[:200, :136].double() + lo[:200, :136].double()
+    assert ((rec - x32.double()).abs() <= x32.double().abs() * 2.0**-21 + 2.0**-24).all()
+    assert (hi[200:] == 0).all() and (hi[:, 136:] == 0).all() and (lo[200:] == 0).all()
+    ht, lt = _split(X, 256, 192, scale=0.125, transpose=True)
+    assert torch.equal(ht, hi.t()) and torch.equal(lt, lo.t())
+
+
+@pytest.mark.parametrize("flags,shape", [
+    (0, (128, 128, 64)),
+    (_lib.GEMM_B_LOWER, (256, 512, 512)),
+    (_lib.GEMM_B_UPPER, (384, 256, 256)),
+    (_lib.GEMM_B_LOWER, (2048, 2048, 2048)),
+    (_lib.GEMM_B_UPPER, (2048, 2048, 2048)),
+])
+def test_gemm_f16x3_planes(flags, shape):
+    _say(f"gemm_f16x3 planes flags={flags} shape={shape}")
+    M, N, K = shape
+    g = torch.Generator().manual_seed(M + 3 * N + K + flags)
+    A = torch.rand(M, K, generator=g, dtype=torch.float64) - 0.3
+    opB = 4.0 * (torch.rand(N, K, generator=g, dtype=torch.float64) - 0.5)
+    if flags & _lib.GEMM_B_LOWER:
+        opB = torch.tril(opB)
+    if flags & _lib.GEMM_B_UPPER:
+        opB = torch.triu(opB)
+    rs = 2.0 ** torch.randint(-3, 4, (M,), generator=g).float()   # per-row scales of A
+    Ah, Al = _split((A * rs.double().unsqueeze(1) * 0.5).to(DEV))
+    Bh, Bl = _split(opB.to(DEV), scale=0.25)
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device=DEV)
+    Ch = torch.empty(M, N, dtype=torch.float16, device=DEV)
+    Cl = torch.empty(M, N, dtype=torch.float16, device=DEV)
+    rsd = rs.to(DEV)
+    _lib.check(_lib.lib().bta_gemm_f16x3(flags, M, N, K, 2.0, _lib.ptr(Ah), _lib.ptr(Al), K,
+                                         _lib.ptr(rsd), 0.5, _lib.ptr(Bh), _lib.ptr(Bl), K, 0.25,
+                                         _lib.ptr(C), N, _lib.ptr(Ch), _lib.ptr(Cl), 0.125,
+                                         _lib.stream()))
+    ref = 2.0 * A @ opB.t()
+    Cc = C.double().cpu()
+    err = ((Cc - ref).abs().max() / ref.abs().max()).item()
+    # 22-bit operands, FP64 sums of 256-long FP32 segments: FP32-rounding-level error;
+    # a dropped hi/lo cross product would show at the 2^-11 level
+    assert err < 2e-6, err
+    # the output planes carry 0.125 C to 2^-22 relative
+    x = (C * 0.125).double()
+    rec = Ch.double() + Cl.double()
+    assert ((rec - x).abs() <= x.abs() * 2.0**-21 + 2.0**-24).all()
+
+
 def _acq_pair(n, d, q, S, precision, seed):
     g = torch.Generator().manual_seed(seed)
     X = torch.rand(n, d, generator=g, dtype=torch.float64)
