@@ -3,48 +3,73 @@
 // with the a_lo b_lo term dropped (2^-22 relative).  A (M x K) and B (N x K) are
 // two FP16 planes each, K-contiguous, written by their producers (the
 // covariance kernel, this GEMM's own split epilogue, the factor set-up), so the
-// main loop is pure data movement plus three v_mfma_f32_16x16x32_f16 per
-// fragment pair.  FP32 accumulators are flushed into FP64 sums every 256 K.
+// main loop is data movement plus three v_mfma_f32_16x16x32_f16 per fragment
+// pair.  FP32 accumulators are flushed into FP64 sums every 256 K.
 //
-// Tile 128 x 128 x 32, 256 threads = 4 waves of 64 x 64, two LDS stages of the
-// four planes (80 KiB, two workgroups per CU), one barrier per K chunk, the
-// next chunk's global loads in flight during the current chunk's MFMAs.  The
+// Tile 128 x BN (BN = 128 or 64), 256 threads = 4 waves of 64 x BN/2, K chunks
+// of 32.  Operands move global -> LDS with direct LDS loads
+// (global_load_lds_dwordx4: no registers, no ds_write), through a ring of four
+// LDS stages, so three chunks are in flight while one is multiplied; one
+// barrier per chunk.  Each plane row (64 bytes) holds its four 16-byte chunks
+// in XOR-swizzled slots (slot = chunk ^ (row / 4 % 4)), which makes both the
+// 1 KiB wave loads and the 16-lane fragment reads bank-conflict free.  The
 // epilogue writes C in FP32 and, optionally, C's own hi/lo planes (scaled by
-// c_scale) for a following split GEMM.  Triangular K ranges and the worker
-// schedule follow gemm_kernel.
+// c_scale) for a following split GEMM.  Triangular K ranges and the snake
+// worker schedule follow gemm_kernel; a cost model picks BN and the worker count.
 #include "bta_common.h"
 #include "gemm.h"
 #include "gemm_tile.h"
 #include "kernels.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 namespace bta {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef int int4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
-constexpr int H2_BM = 128, H2_BN = 128, H2_BK = 64;
-constexpr int H2_PITCH = H2_BK + 8;  // halfs per LDS row: 144 bytes, conflict-free b128 reads
-constexpr int H2_PLANE = H2_BM * H2_PITCH;
-constexpr int H2_STAGE = 4 * H2_PLANE;  // A_hi, A_lo, B_hi, B_lo
-constexpr int H2_SEG = 256;             // K per FP64 flush
-constexpr int H2_F = 4;                 // 16 x 16 fragments per wave side (64 / 16)
-constexpr int H2_ROWS = 256 / (H2_BK / 8);  // plane rows staged per pass (32)
-constexpr int H2_PASS = H2_BM / H2_ROWS;    // passes per plane (4)
+constexpr int H2_BM = 128;
+constexpr int H2_BK = 32;        // K per chunk (one MFMA K-step)
+constexpr int H2_STAGES = 4;     // LDS ring: H2_STAGES - 1 chunks in flight
+constexpr int H2_ROWB = 64;      // bytes per plane row per stage (32 halfs)
+constexpr int H2_SEG = 256;      // K per FP64 flush
 
-// One workgroup per CU (one wave per SIMD): the FP32 + FP64 accumulators of a
-// 64 x 64 wave tile plus fragments and staging need ~350 registers.
+template <int BN>
+struct H2Cfg {
+  static constexpr int WN = BN / 2;                 // wave tile columns
+  static constexpr int FM = 4, FN = WN / 16;        // 16 x 16 fragments per wave
+  static constexpr int PA = H2_BM * H2_ROWB;        // bytes per A plane per stage
+  static constexpr int PB = BN * H2_ROWB;           // bytes per B plane per stage
+  static constexpr int STAGE = 2 * PA + 2 * PB;
+  static constexpr int LA = PA / 1024, LB = PB / 1024;  // 1 KiB wave loads per plane
+  static constexpr int LOADS_W = (2 * LA + 2 * LB) / 4;  // per wave per chunk
+};
+
+// s_waitcnt on the vector-memory counter only (expcnt, lgkmcnt left open)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+template <int BN>
 __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
-  __shared__ __attribute__((aligned(16))) _Float16 lds[2 * H2_STAGE];
-  const int MT = g.M / H2_BM, NT = g.N / H2_BN;
+  using C = H2Cfg<BN>;
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[H2_STAGES * C::STAGE];
+  const int MT = g.M / H2_BM, NT = g.N / BN;
   const int total = MT * NT;
   const int W = gridDim.x, w = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
-  // staging: thread -> (row, 16-byte column) of each plane, H2_PASS rows per plane
-  const int sr = tid / (H2_BK / 8), sk = (tid % (H2_BK / 8)) * 8;
+  // fragment reads: row l16 of a 16-row block, chunk gq in its swizzled slot
+  const int foff = l16 * H2_ROWB + ((gq ^ (l16 >> 2)) << 4);
+  // direct loads: lane -> (row lane/4 of a 16-row block, slot lane%4, source chunk)
+  const int lrow = lane >> 2;
+  const int lchunk = (lane & 3) ^ ((lrow >> 2) & 3);
 
   for (int jj = 0;; ++jj) {
     const int t = (jj & 1) ? (jj + 1) * W - 1 - w : jj * W + w;
@@ -52,112 +77,124 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
     int tm, tn;
     decode_tile(t, g.flags, MT, NT, tm, tn);
     int kb = 0, ke = g.K;
-    if (g.flags & BTA_GEMM_B_LOWER) ke = min(ke, (tn + 1) * H2_BN);
-    if (g.flags & BTA_GEMM_B_UPPER) kb = max(kb, tn * H2_BN);
+    if (g.flags & BTA_GEMM_B_LOWER) ke = min(ke, (tn + 1) * BN);
+    if (g.flags & BTA_GEMM_B_UPPER) kb = max(kb, tn * BN);
     kb = kb / H2_BK * H2_BK;
     ke = (ke + H2_BK - 1) / H2_BK * H2_BK;
-    const int row0 = tm * H2_BM, col0 = tn * H2_BN;
-    const _Float16* src[4] = {g.Ah + (int64_t)(row0 + sr) * g.lda + sk,
-                              g.Al + (int64_t)(row0 + sr) * g.lda + sk,
-                              g.Bh + (int64_t)(col0 + sr) * g.ldb + sk,
-                              g.Bl + (int64_t)(col0 + sr) * g.ldb + sk};
-    const int64_t step[4] = {H2_ROWS * g.lda, H2_ROWS * g.lda, H2_ROWS * g.ldb,
-                             H2_ROWS * g.ldb};
-    int4v st[4 * H2_PASS];
-    auto gload = [&](int k0) {
+    const int row0 = tm * H2_BM, col0 = tn * BN;
+    const int nk = ke > kb ? (ke - kb) / H2_BK : 0;
+
+    // the LOADS_W wave loads of chunk ci into stage ci % H2_STAGES
+    auto issue = [&](int ci) {
+      const int k0 = kb + ci * H2_BK + lchunk * 8;
+      unsigned char* st = lds + (ci % H2_STAGES) * C::STAGE;
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int i = 0; i < H2_PASS; ++i)
-          st[p * H2_PASS + i] = *(const int4v*)(src[p] + i * step[p] + k0);
-    };
-    auto sstore = [&](_Float16* s) {
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int i = 0; i < H2_PASS; ++i)
-          *(int4v*)(s + p * H2_PLANE + (sr + i * H2_ROWS) * H2_PITCH + sk) = st[p * H2_PASS + i];
+      for (int j = 0; j < C::LOADS_W; ++j) {
+        const int gi = wv * C::LOADS_W + j;  // wave-uniform
+        const _Float16* src;
+        int off;
+        if (gi < 2 * C::LA) {
+          const int p = gi / C::LA, rb = gi % C::LA;
+          src = (p ? g.Al : g.Ah) + (int64_t)(row0 + rb * 16 + lrow) * g.lda + k0;
+          off = p * C::PA + rb * 1024;
+        } else {
+          const int gb = gi - 2 * C::LA;
+          const int p = gb / C::LB, rb = gb % C::LB;
+          src = (p ? g.Bl : g.Bh) + (int64_t)(col0 + rb * 16 + lrow) * g.ldb + k0;
+          off = 2 * C::PA + p * C::PB + rb * 1024;
+        }
+        __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(st + off), 16, 0, 0);
+      }
     };
 
-    v4f acc[H2_F][H2_F];
-    double acc64[H2_F][H2_F][4];
+    v4f acc[C::FM][C::FN];
+    double acc64[C::FM][C::FN][4];
 #pragma unroll
-    for (int mi = 0; mi < H2_F; ++mi)
+    for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
-      for (int ni = 0; ni < H2_F; ++ni)
+      for (int ni = 0; ni < C::FN; ++ni)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc64[mi][ni][q] = 0.0;
 
-    const int nk = (ke - kb) / H2_BK;
-    if (nk > 0) {
-      gload(kb);
-      sstore(lds);
-      __syncthreads();
-    }
+#pragma unroll
+    for (int ci = 0; ci < H2_STAGES - 1; ++ci)
+      if (ci < nk) issue(ci);
     for (int it = 0; it < nk; ++it) {
+      // this wave's loads of chunk it have landed once at most
+      // min(H2_STAGES - 2, nk - 1 - it) later chunks remain outstanding
+      const int later = min(H2_STAGES - 2, nk - 1 - it);
+      if (later >= 2) wait_vmcnt<2 * C::LOADS_W>();
+      else if (later == 1) wait_vmcnt<C::LOADS_W>();
+      else wait_vmcnt<0>();
+      // every wave's loads of chunk it landed and every wave is done with chunk
+      // it - 1.  A bare s_barrier: the fence of __syncthreads would also wait for
+      // the loads still in flight; the empty asm statements keep the compiler
+      // from moving LDS accesses across it.
+      __asm__ __volatile__("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      __asm__ __volatile__("" ::: "memory");
+      if (it + H2_STAGES - 1 < nk) issue(it + H2_STAGES - 1);
       if ((it * H2_BK) % H2_SEG == 0) {
 #pragma unroll
-        for (int mi = 0; mi < H2_F; ++mi)
+        for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
-          for (int ni = 0; ni < H2_F; ++ni) acc[mi][ni] = (v4f)(0.0f);
+          for (int ni = 0; ni < C::FN; ++ni) acc[mi][ni] = (v4f)(0.0f);
       }
-      const _Float16* cur = lds + (it & 1) * H2_STAGE;
-      _Float16* nxt = lds + ((it + 1) & 1) * H2_STAGE;
-      if (it + 1 < nk) gload(kb + (it + 1) * H2_BK);
+      const unsigned char* cur = lds + (it % H2_STAGES) * C::STAGE;
+      half8 ah[C::FM], al[C::FM], bh[C::FN], bl[C::FN];
 #pragma unroll
-      for (int ks = 0; ks < H2_BK / 32; ++ks) {
-        half8 ah[H2_F], al[H2_F], bh[H2_F], bl[H2_F];
-        const int ko = ks * 32 + gq * 8;
-#pragma unroll
-        for (int f = 0; f < H2_F; ++f) {
-          const int ra = wy * 64 + f * 16 + l16, rb = wx * 64 + f * 16 + l16;
-          ah[f] = *(const half8*)(cur + ra * H2_PITCH + ko);
-          al[f] = *(const half8*)(cur + H2_PLANE + ra * H2_PITCH + ko);
-          bh[f] = *(const half8*)(cur + 2 * H2_PLANE + rb * H2_PITCH + ko);
-          bl[f] = *(const half8*)(cur + 3 * H2_PLANE + rb * H2_PITCH + ko);
-        }
-        // three passes over the 16 fragments: no back-to-back dependent MFMAs
-#pragma unroll
-        for (int mi = 0; mi < H2_F; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < H2_F; ++ni)
-            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[mi], bh[ni], acc[mi][ni], 0,
-                                                                 0, 0);
-#pragma unroll
-        for (int mi = 0; mi < H2_F; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < H2_F; ++ni)
-            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bl[ni], acc[mi][ni], 0,
-                                                                 0, 0);
-#pragma unroll
-        for (int mi = 0; mi < H2_F; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < H2_F; ++ni)
-            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bh[ni], acc[mi][ni], 0,
-                                                                 0, 0);
+      for (int f = 0; f < C::FM; ++f) {
+        const int o = (wy * 64 + f * 16) * H2_ROWB + foff;
+        ah[f] = *(const half8*)(cur + o);
+        al[f] = *(const half8*)(cur + C::PA + o);
       }
-      if (it + 1 < nk) sstore(nxt);
+#pragma unroll
+      for (int f = 0; f < C::FN; ++f) {
+        const int o = (wx * C::WN + f * 16) * H2_ROWB + foff;
+        bh[f] = *(const half8*)(cur + 2 * C::PA + o);
+        bl[f] = *(const half8*)(cur + 2 * C::PA + C::PB + o);
+      }
+      // three passes over the fragments: no back-to-back dependent MFMAs
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[mi], bh[ni], acc[mi][ni], 0,
+                                                               0, 0);
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bl[ni], acc[mi][ni], 0,
+                                                               0, 0);
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bh[ni], acc[mi][ni], 0,
+                                                               0, 0);
       if (((it + 1) * H2_BK) % H2_SEG == 0 || it + 1 == nk) {
 #pragma unroll
-        for (int mi = 0; mi < H2_F; ++mi)
+        for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
-          for (int ni = 0; ni < H2_F; ++ni)
+          for (int ni = 0; ni < C::FN; ++ni)
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc64[mi][ni][q] += (double)acc[mi][ni][q];
       }
-      __syncthreads();
     }
+    // all waves done reading the ring before the next tile's loads overwrite it
+    __syncthreads();
     const double bs = (double)g.b_scale;
 #pragma unroll
-    for (int mi = 0; mi < H2_F; ++mi)
+    for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int64_t row = row0 + wy * 64 + mi * 16 + Mfma<float>::row(gq, q);
         const double f =
             g.alpha / ((double)g.a_scale * (g.a_row_scale ? (double)g.a_row_scale[row] : 1.0) * bs);
 #pragma unroll
-        for (int ni = 0; ni < H2_F; ++ni) {
-          const int64_t col = col0 + wx * 64 + ni * 16 + l16;
+        for (int ni = 0; ni < C::FN; ++ni) {
+          const int64_t col = col0 + wx * C::WN + ni * 16 + l16;
           const float v = (float)(acc64[mi][ni][q] * f);
           g.C[row * g.ldc + col] = v;
           if (g.Ch) {
@@ -171,58 +208,78 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
   }
 }
 
-// Snake-schedule cost model (as gemm_kernel's) for the 128 x 128 tile.
-static int h2_workers(const H2Args& g) {
-  const int MT = g.M / H2_BM, NT = g.N / H2_BN, total = MT * NT;
+// Snake-schedule cost model: chunk counts per worker, CU time = the sum over
+// the workers it hosts (one workgroup per CU at a time), per-tile fixed cost.
+static double h2_cost(const H2Args& g, int BN, int W) {
+  const int MT = g.M / H2_BM, NT = g.N / BN, total = MT * NT;
   const int ncu = num_cus();
-  std::vector<double> cu(ncu);
-  double best = 1e300;
-  int bestW = 1;
-  const int cands[] = {2 * ncu, 3 * ncu / 2, ncu, 3 * ncu / 4, ncu / 2};
-  for (int c : cands) {
-    const int W = std::max(1, std::min(c, total));
-    std::fill(cu.begin(), cu.end(), 0.0);
-    for (int w = 0; w < W; ++w) {
-      double load = 0.0;
-      for (int j = 0;; ++j) {
-        const int t = (j & 1) ? (j + 1) * W - 1 - w : j * W + w;
-        if (t >= total) break;
-        int tm, tn;
-        decode_tile(t, g.flags, MT, NT, tm, tn);
-        int kb = 0, ke = g.K;
-        if (g.flags & BTA_GEMM_B_LOWER) ke = std::min(ke, (tn + 1) * H2_BN);
-        if (g.flags & BTA_GEMM_B_UPPER) kb = std::max(kb, tn * H2_BN);
-        load += (ke > kb ? ke - kb : 0) + 4 * H2_BK;
-      }
-      // two workgroups share a CU: a CU's time is the larger of its two
-      // workers' loads, or their sum at half rate when both are busy
-      cu[w % ncu] += load;
+  std::vector<double> cu(ncu, 0.0);
+  const double per_chunk = BN == 128 ? 1.0 : 0.55;  // MFMA + LDS time of one K chunk
+  for (int w = 0; w < W; ++w) {
+    double load = 0.0;
+    for (int j = 0;; ++j) {
+      const int t = (j & 1) ? (j + 1) * W - 1 - w : j * W + w;
+      if (t >= total) break;
+      int tm, tn;
+      decode_tile(t, g.flags, MT, NT, tm, tn);
+      int kb = 0, ke = g.K;
+      if (g.flags & BTA_GEMM_B_LOWER) ke = std::min(ke, (tn + 1) * BN);
+      if (g.flags & BTA_GEMM_B_UPPER) kb = std::max(kb, tn * BN);
+      load += per_chunk * (ke > kb ? (ke - kb) / H2_BK : 0) + 6.0;
     }
-    const double t = *std::max_element(cu.begin(), cu.end());
-    if (t < best) {
-      best = t;
-      bestW = W;
+    cu[w % ncu] += load;
+  }
+  return *std::max_element(cu.begin(), cu.end());
+}
+
+struct H2Plan {
+  int bn, workers;
+};
+
+static H2Plan h2_plan(const H2Args& g) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, H2Plan> cache;
+  const auto key = std::make_tuple(g.M, g.N, g.K, g.flags);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const int ncu = num_cus();
+  H2Plan best{128, 1};
+  double bt = 1e300;
+  for (int bn : {128, 64}) {
+    if (g.N % bn) continue;
+    const int total = (g.M / H2_BM) * (g.N / bn);
+    for (int c : {ncu, 3 * ncu / 4, ncu / 2}) {
+      const int W = std::max(1, std::min(c, total));
+      const double t = h2_cost(g, bn, W);
+      if (t < bt) {
+        bt = t;
+        best = {bn, W};
+      }
     }
   }
-  return bestW;
+  std::lock_guard<std::mutex> lk(mu);
+  cache[key] = best;
+  return best;
 }
 
 int gemm_h2(const H2Args& a, hipStream_t s) {
   const uintptr_t al16 = (uintptr_t)a.Ah | (uintptr_t)a.Al | (uintptr_t)a.Bh | (uintptr_t)a.Bl;
-  if (a.M % H2_BM || a.N % H2_BN || a.K % H2_BK || a.M <= 0 || a.N <= 0 || a.K < 0 ||
+  if (a.M % H2_BM || a.N % 64 || a.K % H2_BK || a.M <= 0 || a.N <= 0 || a.K <= 0 ||
       a.lda % 8 || a.ldb % 8 || a.lda < a.K || a.ldb < a.K || a.ldc < a.N || !a.Ah || !a.Al ||
       !a.Bh || !a.Bl || !a.C || (a.Ch && !a.Cl) || (al16 & 15) ||
       (a.flags & ~(BTA_GEMM_B_LOWER | BTA_GEMM_B_UPPER))) {
-    set_error("gemm_h2: M%%128, N%%128, K%%64, lda/ldb%%8 must be 0, operand planes 16-byte "
-              "aligned; flags B_LOWER/B_UPPER only");
+    set_error("gemm_h2: M%%128, N%%64, K%%32, lda/ldb%%8 must be 0, operand planes 16-byte "
+              "aligned, K > 0; flags B_LOWER/B_UPPER only");
     return BTA_ERR_INVALID;
   }
-  if (a.K == 0) {
-    set_error("gemm_h2: K must be positive");
-    return BTA_ERR_INVALID;
-  }
-  const int W = h2_workers(a);
-  hipLaunchKernelGGL(gemm_h2_kernel, dim3(W), dim3(256), 0, s, a);
+  const H2Plan p = h2_plan(a);
+  if (p.bn == 128)
+    hipLaunchKernelGGL(gemm_h2_kernel<128>, dim3(p.workers), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(gemm_h2_kernel<64>, dim3(p.workers), dim3(256), 0, s, a);
   BTA_LAUNCH_CHECK("gemm_h2_kernel");
   return BTA_OK;
 }

@@ -1,212 +1,252 @@
-// Elementwise stages of one qLogNEI value+grad step, tiled through LDS.
+// Elementwise stages of one qLogNEI value+grad step on FP64 MFMA.
 //
-// qstep_covar_kernel (one 32-row x 64-column tile per block):
-//   xs  = (X - center) / l                          (scaled query rows, FP64)
+// qstep_covar_kernel (one 64-row x 64-column tile per block, 16 rows a wave):
+//   xs  = (X - center) / l                          (scaled query rows)
 //   K_x = s k(xs, X_train)                          (stored FP64, FP32, or as FP16
 //                                                    hi/lo planes of 2^e K_x)
 //   Ppart[tile] = K_x[:, tile] [alpha | A][tile, :] (FP64 partial products)
 // qstep_grad_kernel (same tiling), with U = K_x K^-1 and the q x q blocks gq:
 //   g_rc = sum_j gq[r][j] U[q floor(r/q) + j][c] + dext[r] . [alpha|A][c]
 //   w_rc = -g_rc h(|xs_r - t_c|)                    (dK_x/dxs = -h (xs - t))
-//   Dpart[tile][r] = sum_{c in tile} w_rc (xs_r - t_c)
+//   Dpart[tile][r] = xs_r sum_c w_rc - sum_c w_rc t_c
 // qstep_grad_finish_kernel: dX = (sum_tiles Dpart + direct) / l.
 //
-// A block stages its rows, its 64 training points and their [alpha|A] rows
-// (16 columns per pass) in LDS with coalesced loads; each thread evaluates 8
-// kernel entries (one column, 8 rows) in FP64, and the row sums over the
-// tile's columns run from an LDS copy of the tile (8 lanes per row,
-// shuffle-reduced).  Partials over column tiles are summed in a fixed order by
-// their consumer: no atomics, deterministic results.
+// Every contraction is a v_mfma_f64_16x16x4f64: the cross term t_c . x_r of
+// r^2 = |t|^2 + |x|^2 - 2 t.x (the expansion gpytorch uses, clamped at 0), the
+// [alpha|A] products, the restart-block mixing of U, dext [alpha|A]^T, and the
+// contraction of w with [T | 1].  The cross term is formed transposed, D[c][r],
+// so that each lane's four accumulator values K[r = lane%16][c = lane/16 + 4q]
+// are already the A operand of the next product over c (k index = lane/16 + 4q);
+// the VALU only evaluates the kernel functions.  Partials over column tiles are
+// summed in a fixed order by their consumer: no atomics, deterministic results.
 #include "bta_common.h"
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace bta {
 
-constexpr int QS_RT = 32;           // rows per tile
-constexpr int QS_CT = QS_COL_TILE;  // columns (training points) per tile
+constexpr int QS_RT = 64;           // rows per block (16 per wave)
+constexpr int QS_CT = QS_COL_TILE;  // columns (training points) per block
 constexpr int QS_MAXQ = 8;
 
-template <int MD>
-struct QsStage {
-  double xr[QS_RT][MD + 1];
-  double tc[QS_CT][MD + 1];
-  double ec[QS_CT][QS_NE_MAX];
-  double kt[QS_RT][QS_CT + 1];
+__device__ __forceinline__ v4d mfma64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// training points of the tile and their squared norms
+template <int KD>
+struct QsT {
+  double tc[QS_CT][4 * KD + 1];
+  double tn2[QS_CT];
 };
 
-template <int MD>
-__device__ __forceinline__ double qs_dist2(const double* a, const double* b) {
-  double r2 = 0.0;
-#pragma unroll
-  for (int j = 0; j < MD; ++j) {
-    const double df = a[j] - b[j];
-    r2 = fma(df, df, r2);
-  }
-  return r2;
-}
-
-// sum over groups of 8 adjacent lanes
-__device__ __forceinline__ double sum8(double v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  return v;
-}
-
-template <int MD>
-__device__ __forceinline__ void qs_stage_t(QsStage<MD>& sm, const double* T, int64_t n, int d,
+template <int KD>
+__device__ __forceinline__ void qs_stage_t(QsT<KD>& sm, const double* T, int64_t n, int d,
                                            int64_t c0) {
+  constexpr int MD = 4 * KD;
   for (int e = threadIdx.x; e < QS_CT * MD; e += 256) {
     const int c = e / MD, j = e % MD;
     sm.tc[c][j] = (j < d && c0 + c < n) ? T[(c0 + c) * d + j] : 0.0;
   }
+  __syncthreads();
+  if (threadIdx.x < QS_CT) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < MD; ++j) s = fma(sm.tc[threadIdx.x][j], sm.tc[threadIdx.x][j], s);
+    sm.tn2[threadIdx.x] = s;
+  }
+  __syncthreads();
 }
 
-// columns [j0, j0 + nj) of [alpha | A] for the tile's training points
-template <int MD>
-__device__ __forceinline__ void qs_stage_e(QsStage<MD>& sm, const double* E, int64_t n, int ne,
-                                           int j0, int nj, int64_t c0) {
-  for (int e = threadIdx.x; e < QS_CT * nj; e += 256) {
-    const int c = e / nj, j = e % nj;
-    sm.ec[c][j] = c0 + c < n ? E[(c0 + c) * ne + j0 + j] : 0.0;
-  }
+// sum over the four 16-lane groups (same lane % 16)
+__device__ __forceinline__ double sum_groups(double v) {
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
 }
+
+template <typename TO>
+struct alignas(16) QsKStage {
+  TO k[QS_RT][QS_CT + 16 / (int)sizeof(TO)];
+};
 
 // mode: 0 = FP64 K, 1 = FP32 K, 2 = FP16 hi/lo planes of K * kscale
-template <int MD, int MODE>
+template <int KD, int MODE>
 __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
-  __shared__ QsStage<MD> sm;
-  const int tid = threadIdx.x;
+  typedef typename std::conditional<MODE == 0, double,
+                                    typename std::conditional<MODE == 1, float,
+                                                              _Float16>::type>::type TO;
+  __shared__ QsT<KD> sm;
+  __shared__ QsKStage<TO> ks[MODE == 2 ? 2 : 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
   const int ct = blockIdx.x;
   const int d = a.d;
-  const int64_t r0 = (int64_t)blockIdx.y * QS_RT, c0 = (int64_t)ct * QS_CT;
-  for (int e = tid; e < QS_RT * MD; e += 256) {
-    const int r = e / MD, j = e % MD;
+  const int64_t c0 = (int64_t)ct * QS_CT, r0 = (int64_t)blockIdx.y * QS_RT;
+  const int64_t r0w = r0 + wv * 16;
+  const int64_t rl = r0w + l16;  // this lane's row as B operand / D column
+  qs_stage_t<KD>(sm, a.T, a.n, d, c0);
+
+  // scaled row coordinates: lane (g, l16) holds x[rl][4s + g]
+  double xb[KD];
+  double xn2 = 0.0;
+#pragma unroll
+  for (int s = 0; s < KD; ++s) {
+    const int j = 4 * s + g;
     double v = 0.0;
-    if (j < d && r0 + r < a.R) v = (a.X[(r0 + r) * d + j] - a.center[j]) / a.ls[j];
-    sm.xr[r][j] = v;
-    if (ct == 0 && j < d) a.xs_out[(r0 + r) * d + j] = v;
+    if (j < d && rl < a.R) v = (a.X[rl * d + j] - a.center[j]) / a.ls[j];
+    xb[s] = v;
+    xn2 = fma(v, v, xn2);
+    if (ct == 0 && j < d) a.xs_out[rl * d + j] = v;
   }
-  qs_stage_t<MD>(sm, a.T, a.n, d, c0);
-  qs_stage_e<MD>(sm, a.E, a.n, a.ne, 0, min(a.ne, QS_NE_MAX), c0);
-  __syncthreads();
-  const int c = tid & 63, rg = tid >> 6;  // a wave covers the 64 columns of 8 rows
-  const bool cvalid = c0 + c < a.n;
+  xn2 = sum_groups(xn2);
+
+  // K^T blocks: kv[cb][q] = K[rl][c0 + cb*16 + g + 4q]
+  double kv[4][4];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int r = rg + 4 * i;
-    const int64_t gr = r0 + r;
-    double kv = 0.0;
-    if (gr < a.R && cvalid) kv = kern_value(a.kind, qs_dist2<MD>(sm.xr[r], sm.tc[c]), a.s);
-    sm.kt[r][c] = kv;
-    const int64_t o = gr * a.ldk + c0 + c;
-    if (MODE == 0) {
-      ((double*)a.K)[o] = kv;
-    } else if (MODE == 1) {
-      ((float*)a.K)[o] = (float)kv;
-    } else {
-      const float x = (float)(kv * a.kscale);
-      const _Float16 h = (_Float16)x;
-      ((_Float16*)a.K)[o] = h;
-      a.Klo[o] = (_Float16)(x - (float)h);
+  for (int cb = 0; cb < 4; ++cb) {
+    v4d cr = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KD; ++s) cr = mfma64(sm.tc[cb * 16 + l16][4 * s + g], xb[s], cr);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int cl = cb * 16 + g + 4 * q;
+      const double r2 = fmax(sm.tn2[cl] + xn2 - 2.0 * cr[q], 0.0);
+      double k = 0.0;
+      if (c0 + cl < a.n && rl < a.R) k = kern_value(a.kind, r2, a.s);
+      kv[cb][q] = k;
+      if (MODE == 2) {
+        const float x = (float)(k * a.kscale);
+        const _Float16 h = (_Float16)x;
+        ks[0].k[wv * 16 + l16][cl] = h;
+        ks[MODE == 2 ? 1 : 0].k[wv * 16 + l16][cl] = (_Float16)(x - (float)h);
+      } else {
+        ks[0].k[wv * 16 + l16][cl] = (TO)k;
+      }
     }
   }
   __syncthreads();
-  // Ppart[ct][r][j] = sum_c kt[r][c] ec[c][j]: 8 lanes per row, 8 columns each
-  const int r = tid >> 3, part = tid & 7;
-  const int64_t gr = r0 + r;
-  for (int j0 = 0; j0 < a.ne; j0 += QS_NE_MAX) {
-    const int nj = min(a.ne - j0, QS_NE_MAX);
-    if (j0 > 0) {
-      __syncthreads();
-      qs_stage_e<MD>(sm, a.E, a.n, a.ne, j0, nj, c0);
-      __syncthreads();
+  // coalesced 16-byte row stores of the K tile (planes)
+  {
+    constexpr int V = 16 / (int)sizeof(TO);  // elements per 16-byte chunk
+    constexpr int CPR = QS_CT / V;           // chunks per row
+    for (int e = tid; e < QS_RT * CPR; e += 256) {
+      const int r = e / CPR, cc = (e % CPR) * V;
+      const int64_t o = (r0 + r) * a.ldk + c0 + cc;
+      *(int4*)((TO*)a.K + o) = *(const int4*)&ks[0].k[r][cc];
+      if (MODE == 2) *(int4*)(a.Klo + o) = *(const int4*)&ks[MODE == 2 ? 1 : 0].k[r][cc];
     }
-    for (int j = 0; j < nj; ++j) {
-      double s = 0.0;
+  }
+  // Ppart[ct][r][j] = sum_c K[r][c] E[c][j] in chunks of 16 columns j
+  for (int j0 = 0; j0 < a.ne; j0 += 16) {
+    const int jj = j0 + l16;
+    v4d p = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int cc = 0; cc < 8; ++cc)
-        s = fma(sm.kt[r][part * 8 + cc], sm.ec[part * 8 + cc][j], s);
-      s = sum8(s);
-      if (part == 0 && gr < a.R) a.Ppart[((int64_t)ct * a.R + gr) * a.ne + j0 + j] = s;
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t c = c0 + cb * 16 + g + 4 * q;
+        const double e = (jj < a.ne && c < a.n) ? a.E[c * a.ne + jj] : 0.0;
+        p = mfma64(kv[cb][q], e, p);
+      }
+    // D layout: lane (g, l16) reg q = P[r0w + g + 4q][jj]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t r = r0w + g + 4 * q;
+      if (jj < a.ne && r < a.R) a.Ppart[((int64_t)ct * a.R + r) * a.ne + jj] = p[q];
     }
   }
 }
 
-template <int MD, typename TU>
+template <int KD, typename TU>
 __global__ __launch_bounds__(256) void qstep_grad_kernel(QsGradArgs a) {
-  __shared__ QsStage<MD> sm;
-  __shared__ double de[QS_RT][QS_NE_MAX];
-  __shared__ double gqs[QS_RT][QS_MAXQ];
-  const int tid = threadIdx.x;
+  constexpr int NB = (4 * KD + 1 + 15) / 16;  // 16-column blocks of [T | 1]
+  __shared__ QsT<KD> sm;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
   const int ct = blockIdx.x;
-  const int d = a.d, q = a.q;
-  const int64_t r0 = (int64_t)blockIdx.y * QS_RT, c0 = (int64_t)ct * QS_CT;
-  for (int e = tid; e < QS_RT * MD; e += 256) {
-    const int r = e / MD, j = e % MD;
-    sm.xr[r][j] = (j < d && r0 + r < a.R) ? a.xs[(r0 + r) * d + j] : 0.0;
+  const int d = a.d, q = a.q, ne = a.ne;
+  const int64_t c0 = (int64_t)ct * QS_CT, r0 = (int64_t)blockIdx.y * QS_RT;
+  const int64_t r0w = r0 + wv * 16;
+  const int64_t rl = r0w + l16;
+  const bool rvalid = rl < a.R;
+  qs_stage_t<KD>(sm, a.T, a.n, d, c0);
+
+  double xb[KD];
+  double xn2 = 0.0;
+#pragma unroll
+  for (int s = 0; s < KD; ++s) {
+    const int j = 4 * s + g;
+    const double v = (j < d && rvalid) ? a.xs[rl * d + j] : 0.0;
+    xb[s] = v;
+    xn2 = fma(v, v, xn2);
   }
-  for (int e = tid; e < QS_RT * q; e += 256) {
-    const int r = e / q, j = e % q;
-    gqs[r][j] = r0 + r < a.R ? a.gq[(r0 + r) * q + j] : 0.0;
-  }
-  qs_stage_t<MD>(sm, a.T, a.n, d, c0);
-  __syncthreads();
+  xn2 = sum_groups(xn2);
+  // U rows feeding this wave's rows: whole restarts [kr0, kr1)
+  const int64_t kr0 = r0w - r0w % q;
+  int64_t kr1 = ((r0w + 15) / q + 1) * q;
+  if (kr1 > a.R) kr1 = a.R;
+  const int ku = kr1 > kr0 ? (int)((kr1 - kr0 + 3) / 4) : 0;
+  const int64_t rblk = rl - rl % q;  // first row of this lane's restart
   const TU* U = (const TU*)a.U;
-  const int c = tid & 63, rg = tid >> 6;
-  const bool cvalid = c0 + c < a.n;
-  double g[8];
+
+  v4d dacc[NB];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int r = rg + 4 * i;
-    const int64_t gr = r0 + r;
-    double v = 0.0;
-    if (gr < a.R && cvalid) {
-      const int64_t rb = gr - gr % q;
-      for (int j = 0; j < q; ++j) v = fma(gqs[r][j], (double)U[(rb + j) * a.ldu + c0 + c], v);
+  for (int nb = 0; nb < NB; ++nb) dacc[nb] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const int64_t ca = c0 + cb * 16 + l16;  // A-operand column
+    v4d cr = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KD; ++s) cr = mfma64(sm.tc[cb * 16 + l16][4 * s + g], xb[s], cr);
+    // G^T[c][r] = sum_j E[c][j] dext[r][j] + sum_u U[u][c] M[u][r]
+    v4d ga = {0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < ne; j0 += 4) {
+      const int j = j0 + g;
+      const double ea = (j < ne && ca < a.n) ? a.E[ca * ne + j] : 0.0;
+      const double db = (j < ne && rvalid) ? a.dext[rl * ne + j] : 0.0;
+      ga = mfma64(ea, db, ga);
     }
-    g[i] = v;
-  }
-  for (int j0 = 0; j0 < a.ne; j0 += QS_NE_MAX) {
-    const int nj = min(a.ne - j0, QS_NE_MAX);
-    if (j0 > 0) __syncthreads();
-    qs_stage_e<MD>(sm, a.E, a.n, a.ne, j0, nj, c0);
-    for (int e = tid; e < QS_RT * nj; e += 256) {
-      const int r = e / nj, j = e % nj;
-      de[r][j] = r0 + r < a.R ? a.dext[(r0 + r) * a.ne + j0 + j] : 0.0;
+    for (int s = 0; s < ku; ++s) {
+      const int64_t u = kr0 + 4 * s + g;
+      const double ua = (u < kr1 && ca < a.n) ? (double)U[u * a.ldu + ca] : 0.0;
+      const double mb = (rvalid && u < kr1 && u - u % q == rblk) ? a.gq[rl * q + (u - rblk)]
+                                                                   : 0.0;
+      ga = mfma64(ua, mb, ga);
     }
-    __syncthreads();
+    // w^T[c][r] = -G h(r^2); then [T | 1]-contraction over c
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int r = rg + 4 * i;
-      double v = g[i];
-      for (int j = 0; j < nj; ++j) v = fma(de[r][j], sm.ec[c][j], v);
-      g[i] = v;
+    for (int qq = 0; qq < 4; ++qq) {
+      const int cl = cb * 16 + g + 4 * qq;
+      const double r2 = fmax(sm.tn2[cl] + xn2 - 2.0 * cr[qq], 0.0);
+      double w = 0.0;
+      if (c0 + cl < a.n && rvalid) w = -ga[qq] * kern_grad_factor(a.kind, r2, a.s, nullptr);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int j = nb * 16 + l16;
+        const double tb = j < d ? sm.tc[cl][j] : (j == d ? 1.0 : 0.0);
+        dacc[nb] = mfma64(w, tb, dacc[nb]);
+      }
     }
   }
+  // D layout: lane (g, l16) reg qq = S[r0w + g + 4qq][nb*16 + l16], S = w [T | 1]
+  const int src = g * 16 + (d & 15);
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int r = rg + 4 * i;
-    const int64_t gr = r0 + r;
-    double w = 0.0;
-    if (gr < a.R && cvalid)
-      w = -g[i] * kern_grad_factor(a.kind, qs_dist2<MD>(sm.xr[r], sm.tc[c]), a.s, nullptr);
-    sm.kt[r][c] = w;
-  }
-  __syncthreads();
-  // Dpart[ct][r][j] = sum_c w[r][c] (xs_r[j] - t_c[j]): 8 lanes per row
-  const int r = tid >> 3, part = tid & 7;
-  const int64_t gr = r0 + r;
-  double w[8];
+  for (int qq = 0; qq < 4; ++qq) {
+    double sw = 0.0;  // sum_c w[r][c], held in column d
 #pragma unroll
-  for (int cc = 0; cc < 8; ++cc) w[cc] = sm.kt[r][part * 8 + cc];
-  for (int j = 0; j < d; ++j) {
-    const double xj = sm.xr[r][j];
-    double acc = 0.0;
+    for (int nb = 0; nb < NB; ++nb) {
+      const double v = __shfl(dacc[nb][qq], src, 64);
+      if (nb == d / 16) sw = v;
+    }
+    const int64_t r = r0w + g + 4 * qq;
 #pragma unroll
-    for (int cc = 0; cc < 8; ++cc) acc = fma(w[cc], xj - sm.tc[part * 8 + cc][j], acc);
-    acc = sum8(acc);
-    if (part == 0 && gr < a.R) a.Dpart[((int64_t)ct * a.R + gr) * d + j] = acc;
+    for (int nb = 0; nb < NB; ++nb) {
+      const int j = nb * 16 + l16;
+      if (j < d && r < a.R)
+        a.Dpart[((int64_t)ct * a.R + r) * d + j] = a.xs[r * d + j] * sw - dacc[nb][qq];
+    }
   }
 }
 
@@ -231,29 +271,44 @@ static int qs_check(const char* what, int d, int ne, int64_t n, int64_t n_pad, i
   return BTA_OK;
 }
 
+template <int KD>
+static void covar_launch(const QsCovarArgs& a, int mode, dim3 grid, hipStream_t st) {
+  if (mode == 0)
+    hipLaunchKernelGGL((qstep_covar_kernel<KD, 0>), grid, dim3(256), 0, st, a);
+  else if (mode == 1)
+    hipLaunchKernelGGL((qstep_covar_kernel<KD, 1>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((qstep_covar_kernel<KD, 2>), grid, dim3(256), 0, st, a);
+}
+
 int qstep_covar(const QsCovarArgs& a, int mode, hipStream_t st) {
   int rc = qs_check("qstep_covar", a.d, a.ne, a.n, a.n_pad, a.R, a.R_pad);
   if (rc) return rc;
-  if (a.ldk < a.n_pad || (mode == 2 && !a.Klo) || mode < 0 || mode > 2) {
+  if (a.ldk < a.n_pad || a.ldk % 8 || (mode == 2 && !a.Klo) || mode < 0 || mode > 2) {
     set_error("qstep_covar: invalid output layout");
     return BTA_ERR_INVALID;
   }
   const dim3 grid((unsigned)(a.n_pad / QS_CT), (unsigned)(a.R_pad / QS_RT));
-#define QS_LAUNCH(MD_)                                                                   \
-  do {                                                                                   \
-    if (mode == 0)                                                                       \
-      hipLaunchKernelGGL((qstep_covar_kernel<MD_, 0>), grid, dim3(256), 0, st, a);       \
-    else if (mode == 1)                                                                  \
-      hipLaunchKernelGGL((qstep_covar_kernel<MD_, 1>), grid, dim3(256), 0, st, a);       \
-    else                                                                                 \
-      hipLaunchKernelGGL((qstep_covar_kernel<MD_, 2>), grid, dim3(256), 0, st, a);       \
-  } while (0)
-  if (a.d <= 8) QS_LAUNCH(8);
-  else if (a.d <= 16) QS_LAUNCH(16);
-  else QS_LAUNCH(32);
-#undef QS_LAUNCH
+  switch ((a.d + 3) / 4) {
+    case 1: covar_launch<1>(a, mode, grid, st); break;
+    case 2: covar_launch<2>(a, mode, grid, st); break;
+    case 3: covar_launch<3>(a, mode, grid, st); break;
+    case 4: covar_launch<4>(a, mode, grid, st); break;
+    case 5: covar_launch<5>(a, mode, grid, st); break;
+    case 6: covar_launch<6>(a, mode, grid, st); break;
+    case 7: covar_launch<7>(a, mode, grid, st); break;
+    default: covar_launch<8>(a, mode, grid, st); break;
+  }
   BTA_LAUNCH_CHECK("qstep_covar_kernel");
   return BTA_OK;
+}
+
+template <int KD>
+static void grad_launch(const QsGradArgs& a, bool u_f32, dim3 grid, hipStream_t st) {
+  if (u_f32)
+    hipLaunchKernelGGL((qstep_grad_kernel<KD, float>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((qstep_grad_kernel<KD, double>), grid, dim3(256), 0, st, a);
 }
 
 int qstep_grad(const QsGradArgs& a, bool u_f32, const double* extra, const double* ls,
@@ -265,17 +320,16 @@ int qstep_grad(const QsGradArgs& a, bool u_f32, const double* extra, const doubl
     return BTA_ERR_INVALID;
   }
   const dim3 grid((unsigned)(a.n_pad / QS_CT), (unsigned)(a.R_pad / QS_RT));
-#define QG_LAUNCH(MD_)                                                                      \
-  do {                                                                                      \
-    if (u_f32)                                                                              \
-      hipLaunchKernelGGL((qstep_grad_kernel<MD_, float>), grid, dim3(256), 0, st, a);       \
-    else                                                                                    \
-      hipLaunchKernelGGL((qstep_grad_kernel<MD_, double>), grid, dim3(256), 0, st, a);      \
-  } while (0)
-  if (a.d <= 8) QG_LAUNCH(8);
-  else if (a.d <= 16) QG_LAUNCH(16);
-  else QG_LAUNCH(32);
-#undef QG_LAUNCH
+  switch ((a.d + 3) / 4) {
+    case 1: grad_launch<1>(a, u_f32, grid, st); break;
+    case 2: grad_launch<2>(a, u_f32, grid, st); break;
+    case 3: grad_launch<3>(a, u_f32, grid, st); break;
+    case 4: grad_launch<4>(a, u_f32, grid, st); break;
+    case 5: grad_launch<5>(a, u_f32, grid, st); break;
+    case 6: grad_launch<6>(a, u_f32, grid, st); break;
+    case 7: grad_launch<7>(a, u_f32, grid, st); break;
+    default: grad_launch<8>(a, u_f32, grid, st); break;
+  }
   BTA_LAUNCH_CHECK("qstep_grad_kernel");
   const int64_t tot = a.R * a.d;
   hipLaunchKernelGGL(qstep_grad_finish_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
