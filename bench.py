@@ -97,7 +97,7 @@ def gemm_fwd_roofline(acq, X, steps, precision):
                                         _lib.ptr(kx[0]), _lib.ptr(kx[1]), n_pad, None, 1.0,
                                         _lib.ptr(linv[0]), _lib.ptr(linv[1]), n_pad, lscale,
                                         _lib.ptr(vt), n_pad, _lib.ptr(vh[0]), _lib.ptr(vh[1]),
-                                        1.0, _lib.stream()))
+                                        1.0, 0, _lib.stream()))
     else:
         kx = torch.rand(R_pad, n_pad, dtype=dt, device="cuda")
         flags = {"fp64": 0, "mixed_f32": _lib.GEMM_F32 | _lib.GEMM_ACC64}[precision]

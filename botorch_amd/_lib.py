@@ -89,7 +89,7 @@ SIGNATURES = {
                                      _i32, _p, _p, _i64, _p]),
     "bta_gemm_f16x3": (ctypes.c_int, [_i32, _i64, _i64, _i64, _f64, _p, _p, _i64, _p,
                                       ctypes.c_float, _p, _p, _i64, ctypes.c_float, _p, _i64,
-                                      _p, _p, ctypes.c_float, _p]),
+                                      _p, _p, ctypes.c_float, _i32, _p]),
     "bta_potrf_workspace_size": (_sz, [_i64]),
     "bta_potrf": (ctypes.c_int, [_p, _i64, _i64, _p, _i64, _i32, _f64,
                                  ctypes.POINTER(_f64), _p, _sz, _p]),

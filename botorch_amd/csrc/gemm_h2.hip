@@ -1,18 +1,23 @@
 // Split-FP16 GEMM on pre-split operands ("h2"):
-//   C = alpha * (A_hi + A_lo)(B_hi + B_lo)^T / (a_scale * a_row_scale[r] * b_scale)
-// with the a_lo b_lo term dropped (2^-22 relative).  A (M x K) and B (N x K) are
+//   C = alpha * (A_hi + A_lo/2^11)(B_hi + B_lo/2^11)^T / (a_scale a_row_scale[r] b_scale)
+// with the lo x lo term dropped (2^-22 relative).  A plane pair holds
+// hi = fp16(x) and lo = fp16(2^11 (x - hi)): the lo plane stays in FP16's normal
+// range for small x, so every entry keeps ~22 significant bits.  The hi x hi and
+// the two cross products accumulate in separate FP32 sets (the cross set is
+// weighted by 2^-11 at the FP64 flush).  A (M x K) and B (N x K) are
 // two FP16 planes each, K-contiguous, written by their producers (the
 // covariance kernel, this GEMM's own split epilogue, the factor set-up), so the
 // main loop is data movement plus three v_mfma_f32_16x16x32_f16 per fragment
-// pair.  FP32 accumulators are flushed into FP64 sums every 256 K.
+// pair.  FP32 accumulators are flushed into FP64 sums every 256 K (or every
+// seg_chunks x 32 K).
 //
 // Tile 128 x BN (BN = 128 or 64), 256 threads = 4 waves of 64 x BN/2, K chunks
 // of 32.  Operands move global -> LDS with direct LDS loads
 // (global_load_lds_dwordx4: no registers, no ds_write), through a ring of four
 // LDS stages, so three chunks are in flight while one is multiplied; one
 // barrier per chunk.  Each plane row (64 bytes) holds its four 16-byte chunks
-// in XOR-swizzled slots (slot = chunk ^ (row / 4 % 4)), which makes both the
-// 1 KiB wave loads and the 16-lane fragment reads bank-conflict free.  The
+// in XOR-swizzled slots (h2_swizzle), which keeps the fragment reads free of
+// bank conflicts; the 1 KiB wave loads write contiguous LDS.  The
 // epilogue writes C in FP32 and, optionally, C's own hi/lo planes (scaled by
 // c_scale) for a following split GEMM.  Triangular K ranges and the snake
 // worker schedule follow gemm_kernel; a cost model picks BN and the worker count.
@@ -36,7 +41,8 @@ constexpr int H2_BM = 128;
 constexpr int H2_BK = 32;        // K per chunk (one MFMA K-step)
 constexpr int H2_STAGES = 4;     // LDS ring: H2_STAGES - 1 chunks in flight
 constexpr int H2_ROWB = 64;      // bytes per plane row per stage (32 halfs)
-constexpr int H2_SEG = 256;      // K per FP64 flush
+constexpr int H2_SEG = 256;      // default K per FP64 flush
+constexpr double H2_LO = 1.0 / 2048.0;  // weight of the lo planes (stored as 2^11 (x - hi))
 
 template <int BN>
 struct H2Cfg {
@@ -48,6 +54,12 @@ struct H2Cfg {
   static constexpr int LA = PA / 1024, LB = PB / 1024;  // 1 KiB wave loads per plane
   static constexpr int LOADS_W = (2 * LA + 2 * LB) / 4;  // per wave per chunk
 };
+
+// Slot of 16-byte chunk c of plane row r is c ^ h2_swizzle(r % 16).  For a
+// fixed c, rows 0..7 (and 8..15) land in distinct 16-byte slots of a 128-byte
+// bank line and rows 0..15 in distinct slots of a 256-byte line, so fragment
+// reads are conflict free whether the LDS serves 8 or 16 lanes per cycle.
+__device__ __forceinline__ int h2_swizzle(int r) { return ((r >> 1) & 3) ^ ((r >> 3) & 1); }
 
 // s_waitcnt on the vector-memory counter only (expcnt, lgkmcnt left open)
 template <int N>
@@ -66,10 +78,11 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
   // fragment reads: row l16 of a 16-row block, chunk gq in its swizzled slot
-  const int foff = l16 * H2_ROWB + ((gq ^ (l16 >> 2)) << 4);
+  const int foff = l16 * H2_ROWB + ((gq ^ h2_swizzle(l16)) << 4);
   // direct loads: lane -> (row lane/4 of a 16-row block, slot lane%4, source chunk)
   const int lrow = lane >> 2;
-  const int lchunk = (lane & 3) ^ ((lrow >> 2) & 3);
+  const int lchunk = (lane & 3) ^ h2_swizzle(lrow);
+  const int seg = g.seg_chunks > 0 ? g.seg_chunks : H2_SEG / H2_BK;  // chunks per FP64 flush
 
   for (int jj = 0;; ++jj) {
     const int t = (jj & 1) ? (jj + 1) * W - 1 - w : jj * W + w;
@@ -107,7 +120,7 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
       }
     };
 
-    v4f acc[C::FM][C::FN];
+    v4f acc[C::FM][C::FN], accx[C::FM][C::FN];
     double acc64[C::FM][C::FN][4];
 #pragma unroll
     for (int mi = 0; mi < C::FM; ++mi)
@@ -119,7 +132,8 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
 #pragma unroll
     for (int ci = 0; ci < H2_STAGES - 1; ++ci)
       if (ci < nk) issue(ci);
-    for (int it = 0; it < nk; ++it) {
+    // chunk it has landed in LDS for every wave; queue chunk it + H2_STAGES - 1
+    auto sync_issue = [&](int it) {
       // this wave's loads of chunk it have landed once at most
       // min(H2_STAGES - 2, nk - 1 - it) later chunks remain outstanding
       const int later = min(H2_STAGES - 2, nk - 1 - it);
@@ -134,12 +148,10 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
       __builtin_amdgcn_s_barrier();
       __asm__ __volatile__("" ::: "memory");
       if (it + H2_STAGES - 1 < nk) issue(it + H2_STAGES - 1);
-      if ((it * H2_BK) % H2_SEG == 0) {
-#pragma unroll
-        for (int mi = 0; mi < C::FM; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < C::FN; ++ni) acc[mi][ni] = (v4f)(0.0f);
-      }
+    };
+    // acc += A_hi B_hi and accx += A_hi B_lo + A_lo B_hi over chunk it; the lo
+    // planes carry 2^11 (x - hi), so accx is weighted by 2^-11 at the flush
+    auto mma = [&](int it) {
       const unsigned char* cur = lds + (it % H2_STAGES) * C::STAGE;
       half8 ah[C::FM], al[C::FM], bh[C::FN], bl[C::FN];
 #pragma unroll
@@ -159,28 +171,43 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
       for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
         for (int ni = 0; ni < C::FN; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[mi], bh[ni], acc[mi][ni], 0,
-                                                               0, 0);
-#pragma unroll
-      for (int mi = 0; mi < C::FM; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < C::FN; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bl[ni], acc[mi][ni], 0,
-                                                               0, 0);
+          accx[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[mi], bh[ni], accx[mi][ni], 0,
+                                                                0, 0);
 #pragma unroll
       for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
         for (int ni = 0; ni < C::FN; ++ni)
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bh[ni], acc[mi][ni], 0,
                                                                0, 0);
-      if (((it + 1) * H2_BK) % H2_SEG == 0 || it + 1 == nk) {
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni)
+          accx[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bl[ni], accx[mi][ni], 0,
+                                                                0, 0);
+    };
+    auto flush = [&]() {
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc64[mi][ni][q] += (double)acc[mi][ni][q] + H2_LO * (double)accx[mi][ni][q];
+    };
+    for (int it = 0; it < nk; ++it) {
+      sync_issue(it);
+      if (it % seg == 0) {
 #pragma unroll
         for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
-          for (int ni = 0; ni < C::FN; ++ni)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc64[mi][ni][q] += (double)acc[mi][ni][q];
+          for (int ni = 0; ni < C::FN; ++ni) {
+            acc[mi][ni] = (v4f)(0.0f);
+            accx[mi][ni] = (v4f)(0.0f);
+          }
       }
+      mma(it);
+      if ((it + 1) % seg == 0 || it + 1 == nk) flush();
     }
     // all waves done reading the ring before the next tile's loads overwrite it
     __syncthreads();
@@ -201,7 +228,7 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
             const float x = v * g.c_scale;
             const _Float16 h = (_Float16)x;
             g.Ch[row * g.ldc + col] = h;
-            g.Cl[row * g.ldc + col] = (_Float16)(x - (float)h);
+            g.Cl[row * g.ldc + col] = (_Float16)((x - (float)h) * 2048.0f);
           }
         }
       }
@@ -318,7 +345,7 @@ __global__ void split_f16_kernel(const T* __restrict__ X, int64_t ldx, int64_t r
     }
     const _Float16 h = (_Float16)x;
     hi[orow * ldo + ocol] = h;
-    lo[orow * ldo + ocol] = (_Float16)(x - (float)h);
+    lo[orow * ldo + ocol] = (_Float16)((x - (float)h) * 2048.0f);
   }
 }
 
@@ -358,12 +385,12 @@ extern "C" int bta_gemm_f16x3(int32_t flags, int64_t M, int64_t N, int64_t K, do
                               const float* a_row_scale, float a_scale, const void* b_hi,
                               const void* b_lo, int64_t ldb, float b_scale, float* C,
                               int64_t ldc, void* c_hi, void* c_lo, float c_scale,
-                              bta_stream_t stream) {
-  if (M > (1 << 30) || N > (1 << 30) || K > (1 << 30)) {
-    bta::set_error("bta_gemm_f16x3: sizes beyond 2^30");
+                              int32_t k_segment, bta_stream_t stream) {
+  if (M > (1 << 30) || N > (1 << 30) || K > (1 << 30) || k_segment < 0 || k_segment % 32) {
+    bta::set_error("bta_gemm_f16x3: sizes beyond 2^30 or k_segment not a multiple of 32");
     return BTA_ERR_INVALID;
   }
-  bta::H2Args a;
+  bta::H2Args a{};
   a.Ah = (const _Float16*)a_hi;
   a.Al = (const _Float16*)a_lo;
   a.Bh = (const _Float16*)b_hi;
@@ -383,5 +410,6 @@ extern "C" int bta_gemm_f16x3(int32_t flags, int64_t M, int64_t N, int64_t K, do
   a.K = (int)K;
   a.flags = flags;
   a.alpha = alpha;
+  a.seg_chunks = k_segment / 32;
   return bta::gemm_h2(a, (hipStream_t)stream);
 }

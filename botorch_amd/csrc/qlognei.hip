@@ -52,12 +52,15 @@ struct McArgs {
 
 // LDS layout (doubles) shared by host sizing and the kernel.
 struct McLayout {
-  int xq, red, mu, dmu, sqq, lbr, ga, lnew, dlnew, dsqb, u, G, scal, total;
+  int xq, red, mu, dmu, sqq, lbr, ga, lnew, dlnew, dsqb, u, G, scal, pv, vv, hh, total;
   __host__ __device__ McLayout(int Q, int n_b, int S) {
     const int m = n_b + Q;
     int o = 0;
     xq = o; o += Q * MC_MAXD;
     red = o; o += 4 * 64;
+    pv = o; o += Q * (n_b + 1);
+    vv = o; o += Q * (Q + 1) / 2;
+    hh = o; o += Q * m;
     mu = o; o += Q;
     dmu = o; o += Q;
     sqq = o; o += Q * Q;
@@ -73,30 +76,30 @@ struct McLayout {
   }
 };
 
-// P[r][j] as the fixed-order sum of its column-tile partials
-__device__ __forceinline__ double mc_p(const McArgs& a, int64_t r, int j) {
-  const double* p = a.P + r * (a.n_b + 1) + j;
-  double s = 0.0;
-  for (int t = 0; t < a.p_parts; ++t) s += p[t * a.p_stride];
-  return s;
-}
-
+// log(tau * (softplus(x/tau) + 0.1 / (1 + (x/tau)^2))) and its derivative
+// (utils/safe_math.py:298-325), evaluated in TM (FP64, or FP32 for the mixed
+// modes' sample arithmetic) from the FP64 difference x.
+template <typename TM>
 __device__ __forceinline__ void log_fatplus_grad(double x, double tau, double* v, double* dv) {
-  // log(tau * (softplus(x/tau) + 0.1 / (1 + (x/tau)^2)))  utils/safe_math.py:298-325
-  const double z = x / tau;
-  double sp, sig;
-  if (z > 20.0) {  // torch softplus threshold
+  const TM z = (TM)(x / tau);
+  TM sp, sig;
+  if (z > (TM)20) {  // torch softplus threshold
     sp = z;
-    sig = 1.0;
+    sig = (TM)1;
   } else {
-    const double e = exp(z);
+    const TM e = exp(z);
     sp = log1p(e);
-    sig = e / (1.0 + e);
+    sig = e / ((TM)1 + e);
   }
-  const double cz = 1.0 / (1.0 + z * z);
-  const double f = sp + 0.1 * cz;
-  *v = log(tau * f);
-  *dv = (sig - 0.2 * z * cz * cz) / (f * tau);
+  const TM z2 = (TM)1 + z * z;
+  const TM cz = (TM)1 / z2;
+  const TM f = sp + (TM)0.1 * cz;
+  const TM tt = (TM)tau;
+  *v = (double)log(tt * f);
+  // dv = (sig - 0.2 z cz^2) / (f tau), with cz / f = 1 / (sp (1 + z^2) + 0.1) so
+  // that no intermediate underflows for large |z|
+  const TM cf = sp > (TM)0 ? (TM)1 / (sp * z2 + (TM)0.1) : (TM)10;
+  *dv = (double)((sig / f - (TM)0.2 * z * cz * cf) / tt);
 }
 
 // block-wide reduction of NV values (NV <= 64); every thread gets the result
@@ -195,43 +198,62 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   for (int i = tid; i < Q * d; i += 256) xq[(i / d) * MC_MAXD + i % d] = a.xs[(r0 + i / d) * d + i % d];
   if (tid == 0) scal[1] = 0.0;
 
-  // ---- Sigma_qq accumulation: V_b V_b^T over the n_pad columns -----------
+  // ---- P rows (fixed-order sums of their column-tile partials, one value per
+  // wave task, partials across the lanes) and V_b V_b^T over the n_pad
+  // columns of V (16-byte loads, all issued before the products) ----------
   constexpr int NQ = Q * (Q + 1) / 2;
+  double* pv = sm + lay.pv;  // Q x (n_b + 1): P[r0 + i][j]
+  double* vv = sm + lay.vv;  // packed lower triangle of V_b V_b^T
   {
+    const int ne = n_b + 1;
+    for (int t = wv; t < Q * ne; t += 4) {
+      double s = 0.0;
+      const double* p = a.P + (r0 + t / ne) * ne + t % ne;
+      for (int pt = lane; pt < a.p_parts; pt += 64) s += p[pt * a.p_stride];
+      s = wave_sum(s);
+      if (lane == 0) pv[t] = s;
+    }
+    constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
+    typedef TV vec_t __attribute__((ext_vector_type(VW)));
     double acc[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) acc[i] = 0.0;
-    for (int k = tid; k < a.n_pad; k += 256) {
-      double v[Q];
+#pragma unroll 2
+    for (int k = tid * VW; k < a.n_pad; k += 256 * VW) {
+      vec_t v[Q];
 #pragma unroll
-      for (int i = 0; i < Q; ++i) v[i] = (double)vt[(r0 + i) * a.ldv + k];
-      int idx = 0;
+      for (int i = 0; i < Q; ++i) v[i] = *(const vec_t*)(vt + (r0 + i) * a.ldv + k);
 #pragma unroll
-      for (int i = 0; i < Q; ++i)
+      for (int e = 0; e < VW; ++e) {
+        int idx = 0;
 #pragma unroll
-        for (int j = 0; j <= i; ++j) {
-          acc[idx] = fma(v[i], v[j], acc[idx]);
-          ++idx;
-        }
+        for (int i = 0; i < Q; ++i)
+#pragma unroll
+          for (int j = 0; j <= i; ++j) {
+            acc[idx] = fma((double)v[i][e], (double)v[j][e], acc[idx]);
+            ++idx;
+          }
+      }
     }
     double tot[NQ];
     block_sum<NQ>(acc, red, tot);
-    if (tid < Q * Q) {
-      const int i = tid / Q, j = tid % Q;
-      const int ii = i > j ? i : j, jj = i > j ? j : i;
-      const double vv = tot[ii * (ii + 1) / 2 + jj];
-      const double kqq = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d),
-                                    a.outputscale);
-      sqq[i * Q + j] = ys2 * (kqq - vv);
-    }
-    if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + mc_p(a, r0 + tid, 0));
+    if (tid < NQ) vv[tid] = tot[tid];
   }
+  __syncthreads();
+  if (tid < Q * Q) {
+    const int i = tid / Q, j = tid % Q;
+    const int ii = i > j ? i : j, jj = i > j ? j : i;
+    const double kqq = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d),
+                                  a.outputscale);
+    sqq[i * Q + j] = ys2 * (kqq - vv[ii * (ii + 1) / 2 + jj]);
+  }
+  if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + pv[tid * (n_b + 1)]);
   // ---- Sigma_qb into lnew[:, :n_b] ----------------------------------------
   for (int e = tid; e < Q * n_b; e += 256) {
     const int i = e / n_b, kb = e % n_b;
     const double kqb = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d),
                                   a.outputscale);
-    lnew[i * m + kb] = ys2 * (kqb - mc_p(a, r0 + i, 1 + kb));
+    lnew[i * m + kb] = ys2 * (kqb - pv[i * (n_b + 1) + 1 + kb]);
   }
   __syncthreads();
 
@@ -300,8 +322,12 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   __syncthreads();
 
   // ---- samples, log improvement, fat-max over q ---------------------------
+  // The per-sample transcendental arithmetic runs in TM: FP64 in the FP64
+  // mode, FP32 in the mixed modes (samples and best_f stay FP64).
+  typedef TV TM;
   const double tr = a.tau_relu, tm = a.tau_max;
   bool bad = false;
+#pragma unroll 2
   for (int s = tid; s < S; s += 256) {
     double y[Q];
 #pragma unroll
@@ -317,26 +343,26 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       bad |= !isfinite(y[i]);
-      log_fatplus_grad(y[i] - bf, tr, &v[i], &dv[i]);
+      log_fatplus_grad<TM>(y[i] - bf, tr, &v[i], &dv[i]);
       M = fmax(M, v[i]);
     }
-    double P = 0.0, sdp = 0.0, dp[Q];
+    TM P = 0, sdp = 0, dp[Q];
     int nties = 0;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const double z = (M - v[i]) / tm;
-      const double den = 2.0 + 2.0 * z + z * z;
-      P += 2.0 / den;
-      dp[i] = -2.0 * (2.0 + 2.0 * z) / (den * den);
+      const TM z = (TM)((M - v[i]) / tm);
+      const TM den = (TM)2 + (TM)2 * z + z * z;
+      P += (TM)2 / den;
+      dp[i] = (TM)-2 * ((TM)2 + (TM)2 * z) / (den * den);
       sdp += dp[i];
       nties += (v[i] == M);
     }
-    uu[s] = M + tm * log(P);
-    const double cm = (1.0 + sdp / P) / (double)nties;
+    uu[s] = M + tm * (double)log(P);
+    const TM cm = ((TM)1 + sdp / P) / (TM)nties;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const double du = (v[i] == M ? cm : 0.0) - dp[i] / P;
-      G[s * Q + i] = du * dv[i];
+      const TM du = (v[i] == M ? cm : (TM)0) - dp[i] / P;
+      G[s * Q + i] = (double)du * dv[i];
     }
   }
   if (bad) scal[1] = 1.0;  // every writer stores the same value
@@ -358,6 +384,46 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   }
   if (!a.grad) return;
 
+  // ---- dmu, d[Lbl | Lbr] = (w G)^T Z with the logmeanexp weights w --------
+  constexpr int MREG = 32 / Q - 1;  // per-thread accumulators while m <= MREG
+  if (m <= MREG) {
+    constexpr int NV = Q * (MREG + 1);
+    double acc[NV];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) acc[e] = 0.0;
+    for (int s = tid; s < S; s += 256) {
+      const double w = exp(uu[s] - Mu) / tsum[0];
+      double gw[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        gw[i] = G[s * Q + i] * w;
+        acc[i * (MREG + 1) + MREG] += gw[i];
+      }
+#pragma unroll
+      for (int k = 0; k < MREG; ++k) {
+        if (k < m) {
+          const double z = a.Zt[(int64_t)k * S + s];
+#pragma unroll
+          for (int i = 0; i < Q; ++i)
+            acc[i * (MREG + 1) + k] = fma(gw[i], z, acc[i * (MREG + 1) + k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+      const double sw = wave_sum(acc[e]);
+      if (lane == 0) red[wv * 64 + e] = sw;
+    }
+    __syncthreads();
+    for (int e = tid; e < Q * (m + 1); e += 256) {
+      const int i = e / (m + 1), k = e % (m + 1);
+      const int idx = i * (MREG + 1) + (k == m ? MREG : k);
+      const double tot = red[idx] + red[64 + idx] + red[128 + idx] + red[192 + idx];
+      if (k == m) dmu[i] = tot;
+      else dlnew[i * m + k] = k < n_b + i + 1 ? tot : 0.0;
+    }
+    __syncthreads();
+  } else {
   for (int s = tid; s < S; s += 256) {
     const double w = exp(uu[s] - Mu) / tsum[0];
 #pragma unroll
@@ -365,7 +431,7 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   }
   __syncthreads();
 
-  // ---- dmu, d[Lbl | Lbr] = G^T Z: one wave per (i, k) --------------------
+  // one wave per (i, k) for wide baselines
   for (int task = wv; task < Q * (m + 1); task += 4) {
     const int i = task / (m + 1), k = task % (m + 1);
     if (k < m && k >= n_b + i + 1) {  // upper part of Lbr: no dependence
@@ -386,6 +452,7 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
     }
   }
   __syncthreads();
+  }
 
   // ---- Cholesky backward of Lbr (torch's formula), gA symmetric ----------
   if (tid == 0) {
@@ -462,21 +529,24 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
     a.dext[(r0 + i) * (n_b + 1) + k] = k == 0 ? ys * dmu[i] : -ys2 * dsqb[i * n_b + k - 1];
   }
   // ---- direct input gradient through K(X, X) and K(X, X_b) (scaled) ------
+  // radial factors first (one thread each), then the d coordinates
+  double* hh = sm + lay.hh;  // Q x m: h(x_i, x_i2) for i2 < Q, h(x_i, b_kb) after
+  for (int e = tid; e < Q * m; e += 256) {
+    const int i = e / m, j2 = e % m;
+    const double* xi = xq + i * MC_MAXD;
+    const double* x2 = j2 < Q ? xq + j2 * MC_MAXD : a.bs + (j2 - Q) * d;
+    hh[e] = j2 == i ? 0.0
+                    : kern_grad_factor(a.kind, sq_dist(xi, x2, d), a.outputscale, nullptr);
+  }
+  __syncthreads();
   for (int e = tid; e < Q * d; e += 256) {
     const int i = e / d, j = e % d;
-    const double* xi = xq + i * MC_MAXD;
+    const double xij = xq[i * MC_MAXD + j];
     double acc = 0.0;
-    for (int i2 = 0; i2 < Q; ++i2) {
-      if (i2 == i) continue;
-      const double* x2 = xq + i2 * MC_MAXD;
-      const double h = kern_grad_factor(a.kind, sq_dist(xi, x2, d), a.outputscale, nullptr);
-      acc -= 2.0 * ys2 * gA[i * Q + i2] * h * (xi[j] - x2[j]);
-    }
-    for (int kb = 0; kb < n_b; ++kb) {
-      const double* x2 = a.bs + kb * d;
-      const double h = kern_grad_factor(a.kind, sq_dist(xi, x2, d), a.outputscale, nullptr);
-      acc -= ys2 * dsqb[i * n_b + kb] * h * (xi[j] - x2[j]);
-    }
+    for (int i2 = 0; i2 < Q; ++i2)
+      acc -= 2.0 * ys2 * gA[i * Q + i2] * hh[i * m + i2] * (xij - xq[i2 * MC_MAXD + j]);
+    for (int kb = 0; kb < n_b; ++kb)
+      acc -= ys2 * dsqb[i * n_b + kb] * hh[i * m + Q + kb] * (xij - a.bs[kb * d + j]);
     a.dxs_extra[(r0 + i) * d + j] = acc;
   }
 }
@@ -609,7 +679,7 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   if (rc) return rc;
   // V = K_x L^-T  (L^-1 lower: the K-range of column tile j stops at its end)
   if (split) {
-    H2Args h;
+    H2Args h{};
     h.Ah = kh;
     h.Al = kh + plane;
     h.Bh = lh;
@@ -638,7 +708,7 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   // U = V L^-1 = K_x K^-1 into kx (K x N view of L^-1: rows k >= column n)
   if (grad) {
     if (split) {
-      H2Args h;
+      H2Args h{};
       h.Ah = vh;
       h.Al = vh + plane;
       h.Bh = lh + 2 * lplane;

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
         const float x = (float)(k * a.kscale);
         const _Float16 h = (_Float16)x;
         ks[0].k[wv * 16 + l16][cl] = h;
-        ks[MODE == 2 ? 1 : 0].k[wv * 16 + l16][cl] = (_Float16)(x - (float)h);
+        ks[MODE == 2 ? 1 : 0].k[wv * 16 + l16][cl] = (_Float16)((x - (float)h) * 2048.0f);
       } else {
         ks[0].k[wv * 16 + l16][cl] = (TO)k;
       }
@@ -250,14 +250,23 @@ __global__ __launch_bounds__(256) void qstep_grad_kernel(QsGradArgs a) {
   }
 }
 
+// Four lanes per element, each summing every fourth partial (8 loads in
+// flight), then combined in a fixed order: deterministic.
 __global__ void qstep_grad_finish_kernel(const double* __restrict__ Dpart, int nct, int64_t R,
                                          int d, const double* __restrict__ extra,
                                          const double* __restrict__ ls, double* __restrict__ dX) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= R * d) return;
-  double acc = extra ? extra[e] : 0.0;
-  for (int t = 0; t < nct; ++t) acc += Dpart[(int64_t)t * R * d + e];
-  dX[e] = acc / ls[(int)(e % d)];
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = gid >> 2;
+  const int part = (int)(gid & 3);
+  const int64_t tot = R * d;
+  double acc = 0.0;
+  if (e < tot) {
+#pragma unroll 8
+    for (int t = part; t < nct; t += 4) acc += Dpart[(int64_t)t * tot + e];
+  }
+  acc += __shfl_xor(acc, 1, 64);
+  acc += __shfl_xor(acc, 2, 64);
+  if (part == 0 && e < tot) dX[e] = (acc + (extra ? extra[e] : 0.0)) / ls[(int)(e % d)];
 }
 
 static int qs_check(const char* what, int d, int ne, int64_t n, int64_t n_pad, int64_t R,
@@ -332,8 +341,8 @@ int qstep_grad(const QsGradArgs& a, bool u_f32, const double* extra, const doubl
   }
   BTA_LAUNCH_CHECK("qstep_grad_kernel");
   const int64_t tot = a.R * a.d;
-  hipLaunchKernelGGL(qstep_grad_finish_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                     st, a.Dpart, (int)(a.n_pad / QS_CT), a.R, a.d, extra, ls, dX);
+  hipLaunchKernelGGL(qstep_grad_finish_kernel, dim3((unsigned)((4 * tot + 255) / 256)), dim3(256),
+                     0, st, a.Dpart, (int)(a.n_pad / QS_CT), a.R, a.d, extra, ls, dX);
   BTA_LAUNCH_CHECK("qstep_grad_finish_kernel");
   return BTA_OK;
 }

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BTA_ABI_VERSION 3
+#define BTA_ABI_VERSION 4
 
 enum {
   BTA_OK = 0,
@@ -43,7 +43,9 @@ enum { BTA_KERNEL_MATERN52 = 0, BTA_KERNEL_RBF = 1 };
 /* FP64: everything FP64.  MIXED: the two L^-1 contractions on split-FP16 MFMA
  * (three FP16 products of hi/lo operand planes, FP64 segment sums).  MIXED_F32:
  * the contractions on FP32 MFMA with FP64 segment sums.  Factors, kernel
- * values, [alpha | A] products and the MC stage are FP64 in every mode. */
+ * values, [alpha | A] products, posterior moments and samples are FP64 in every
+ * mode; the mixed modes evaluate the per-sample log-improvement / fat-max
+ * transcendentals in FP32. */
 enum { BTA_PREC_FP64 = 0, BTA_PREC_MIXED = 1, BTA_PREC_MIXED_F32 = 2 };
 
 /* per-restart status bits written by bta_qlognei_value_grad */
@@ -117,7 +119,8 @@ int bta_gemm(int32_t flags, int64_t M, int64_t N, int64_t K, double alpha, const
 
 /* FP16 hi/lo planes of scale * X (X double when src_f64, else float; rows x
  * cols, row stride ldx), or of scale * X^T when transpose != 0: hi = fp16(x),
- * lo = fp16(x - hi) with x = fp32(scale * X).  Output rows_pad x cols_pad
+ * lo = fp16(2^11 (x - hi)) with x = fp32(scale * X), so x ~ hi + 2^-11 lo with
+ * lo in FP16's normal range for |x| >= 2^-14.  Output rows_pad x cols_pad
  * (cols_pad x rows_pad transposed), row stride ldo, zero in the padding.
  * Operand format of bta_gemm_f16x3; no counterpart in the reference (an
  * internal building block of the split-FP16 contractions). */
@@ -125,18 +128,21 @@ int bta_split_f16(int32_t src_f64, const void* X, int64_t ldx, int64_t rows, int
                   int64_t rows_pad, int64_t cols_pad, float scale, int32_t transpose,
                   void* hi, void* lo, int64_t ldo, bta_stream_t stream);
 
-/* C = alpha (A_hi + A_lo)(B_hi + B_lo)^T / (a_scale * a_row_scale[m] * b_scale)
- * without the lo*lo term, on FP16 MFMA with FP64 sums of 256-long K segments.
+/* C = alpha (A_hi + 2^-11 A_lo)(B_hi + 2^-11 B_lo)^T / (a_scale a_row_scale[m] b_scale)
+ * without the lo*lo term (planes as bta_split_f16 writes them), on FP16 MFMA
+ * with FP64 sums of FP32 K segments.
  * A planes M x K (stride lda), B planes N x K (stride ldb), FP16; C float M x N
  * (stride ldc).  When c_hi is not NULL the planes of c_scale * C are also
- * written (stride ldc).  flags: BTA_GEMM_B_LOWER / BTA_GEMM_B_UPPER.  M, N
- * multiples of 128, K of 32, lda/ldb of 8; a_row_scale may be NULL.  No
- * counterpart in the reference: the contraction kernel of BTA_PREC_MIXED. */
+ * written (stride ldc).  flags: BTA_GEMM_B_LOWER / BTA_GEMM_B_UPPER.  M
+ * multiple of 128, N of 64, K of 32, lda/ldb of 8; a_row_scale may be NULL.
+ * k_segment: K per FP32 accumulation segment before the FP64 add (a multiple
+ * of 32; 0 selects 256).  No counterpart in the reference: the contraction
+ * kernel of BTA_PREC_MIXED. */
 int bta_gemm_f16x3(int32_t flags, int64_t M, int64_t N, int64_t K, double alpha,
                    const void* a_hi, const void* a_lo, int64_t lda, const float* a_row_scale,
                    float a_scale, const void* b_hi, const void* b_lo, int64_t ldb,
                    float b_scale, float* C, int64_t ldc, void* c_hi, void* c_lo,
-                   float c_scale, bta_stream_t stream);
+                   float c_scale, int32_t k_segment, bta_stream_t stream);
 
 size_t bta_potrf_workspace_size(int64_t n_pad);
 
