@@ -2,25 +2,28 @@
 //   C = alpha * (A_hi + A_lo/2^11)(B_hi + B_lo/2^11)^T / (a_scale a_row_scale[r] b_scale)
 // with the lo x lo term dropped (2^-22 relative).  A plane pair holds
 // hi = fp16(x) and lo = fp16(2^11 (x - hi)): the lo plane stays in FP16's normal
-// range for small x, so every entry keeps ~22 significant bits.  The hi x hi and
-// the two cross products accumulate in separate FP32 sets (the cross set is
-// weighted by 2^-11 at the FP64 flush).  A (M x K) and B (N x K) are
-// two FP16 planes each, K-contiguous, written by their producers (the
-// covariance kernel, this GEMM's own split epilogue, the factor set-up), so the
-// main loop is data movement plus three v_mfma_f32_16x16x32_f16 per fragment
-// pair.  FP32 accumulators are flushed into FP64 sums every 256 K (or every
-// seg_chunks x 32 K).
+// range for small x, so every entry keeps ~22 significant bits.  A (M x K) and
+// B (N x K) are two FP16 planes each, K-contiguous, written by their producers
+// (the covariance kernel, this GEMM's own split epilogue, the factor set-up),
+// so the main loop is data movement plus three v_mfma_f32_16x16x32_f16 per
+// fragment pair.  hi x hi and the two cross products accumulate in separate
+// FP32 sets (the cross set weighted by 2^-11 when combined), either over the
+// whole K range (default) or in FP32 segments of k_segment added into FP64
+// sums.  With the lo planes in range, the operand representation dominates
+// the error: FP32 accumulation over K = 2048 leaves V V^T as accurate as FP64
+// segment sums do, so the default path keeps no FP64 accumulators and runs
+// two workgroups (two waves per SIMD) on each CU.
 //
 // Tile 128 x BN (BN = 128 or 64), 256 threads = 4 waves of 64 x BN/2, K chunks
 // of 32.  Operands move global -> LDS with direct LDS loads
-// (global_load_lds_dwordx4: no registers, no ds_write), through a ring of four
-// LDS stages, so three chunks are in flight while one is multiplied; one
-// barrier per chunk.  Each plane row (64 bytes) holds its four 16-byte chunks
-// in XOR-swizzled slots (h2_swizzle), which keeps the fragment reads free of
-// bank conflicts; the 1 KiB wave loads write contiguous LDS.  The
-// epilogue writes C in FP32 and, optionally, C's own hi/lo planes (scaled by
-// c_scale) for a following split GEMM.  Triangular K ranges and the snake
-// worker schedule follow gemm_kernel; a cost model picks BN and the worker count.
+// (global_load_lds_dwordx4: no registers, no ds_write) through a ring of LDS
+// stages with STAGES - 1 chunks in flight; one barrier per chunk.  Each plane
+// row (64 bytes) holds its four 16-byte chunks in XOR-swizzled slots
+// (h2_swizzle), which keeps the fragment reads free of bank conflicts; the
+// 1 KiB wave loads write contiguous LDS.  The epilogue writes C in FP32 and,
+// optionally, C's own hi/lo planes (scaled by c_scale) for a following split
+// GEMM.  Triangular K ranges follow gemm_kernel; a cost model picks BN and the
+// worker count, and workers pair heavy with light tiles on each CU.
 #include "bta_common.h"
 #include "gemm.h"
 #include "gemm_tile.h"
@@ -39,12 +42,10 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 constexpr int H2_BM = 128;
 constexpr int H2_BK = 32;        // K per chunk (one MFMA K-step)
-constexpr int H2_STAGES = 4;     // LDS ring: H2_STAGES - 1 chunks in flight
 constexpr int H2_ROWB = 64;      // bytes per plane row per stage (32 halfs)
-constexpr int H2_SEG = 256;      // default K per FP64 flush
 constexpr double H2_LO = 1.0 / 2048.0;  // weight of the lo planes (stored as 2^11 (x - hi))
 
-template <int BN>
+template <int BN, bool ACC64>
 struct H2Cfg {
   static constexpr int WN = BN / 2;                 // wave tile columns
   static constexpr int FM = 4, FN = WN / 16;        // 16 x 16 fragments per wave
@@ -53,6 +54,9 @@ struct H2Cfg {
   static constexpr int STAGE = 2 * PA + 2 * PB;
   static constexpr int LA = PA / 1024, LB = PB / 1024;  // 1 KiB wave loads per plane
   static constexpr int LOADS_W = (2 * LA + 2 * LB) / 4;  // per wave per chunk
+  // FP64 sums: one workgroup per CU, 4 stages; FP32 only: two per CU
+  static constexpr int STAGES = ACC64 ? 4 : (BN == 64 ? 3 : 2);
+  static constexpr int WG_PER_CU = ACC64 ? 1 : 2;
 };
 
 // Slot of 16-byte chunk c of plane row r is c ^ h2_swizzle(r % 16).  For a
@@ -68,10 +72,22 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
-template <int BN>
-__global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
-  using C = H2Cfg<BN>;
-  __shared__ __attribute__((aligned(1024))) unsigned char lds[H2_STAGES * C::STAGE];
+// Tile of position p in round j of W workers: rounds alternate direction
+// (snake), and within a round the second half of the workers takes the tiles
+// in reverse, so workers p and p + W/2 (one CU when two workgroups share it)
+// hold a heavy and a light tile.
+__host__ __device__ inline int h2_tile_of(int j, int p, int W) {
+  const int h = W / 2;
+  int q = (W > 1 && p >= h) ? W - 1 - (p - h) : p;
+  if (j & 1) q = W - 1 - q;
+  return j * W + q;
+}
+
+template <int BN, bool ACC64>
+__global__ __launch_bounds__(256, (H2Cfg<BN, ACC64>::WG_PER_CU)) void gemm_h2_kernel(H2Args g) {
+  using C = H2Cfg<BN, ACC64>;
+  constexpr int S = C::STAGES;
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[S * C::STAGE];
   const int MT = g.M / H2_BM, NT = g.N / BN;
   const int total = MT * NT;
   const int W = gridDim.x, w = blockIdx.x;
@@ -82,11 +98,12 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
   // direct loads: lane -> (row lane/4 of a 16-row block, slot lane%4, source chunk)
   const int lrow = lane >> 2;
   const int lchunk = (lane & 3) ^ h2_swizzle(lrow);
-  const int seg = g.seg_chunks > 0 ? g.seg_chunks : H2_SEG / H2_BK;  // chunks per FP64 flush
+  const int seg = ACC64 ? (g.seg_chunks > 0 ? g.seg_chunks : 8) : 0;  // chunks per FP64 add
 
   for (int jj = 0;; ++jj) {
-    const int t = (jj & 1) ? (jj + 1) * W - 1 - w : jj * W + w;
-    if (t >= total) break;
+    const int t = h2_tile_of(jj, w, W);
+    if (jj * W >= total) break;
+    if (t >= total) continue;
     int tm, tn;
     decode_tile(t, g.flags, MT, NT, tm, tn);
     int kb = 0, ke = g.K;
@@ -97,10 +114,10 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
     const int row0 = tm * H2_BM, col0 = tn * BN;
     const int nk = ke > kb ? (ke - kb) / H2_BK : 0;
 
-    // the LOADS_W wave loads of chunk ci into stage ci % H2_STAGES
+    // the LOADS_W wave loads of chunk ci into stage ci % S
     auto issue = [&](int ci) {
       const int k0 = kb + ci * H2_BK + lchunk * 8;
-      unsigned char* st = lds + (ci % H2_STAGES) * C::STAGE;
+      unsigned char* st = lds + (ci % S) * C::STAGE;
 #pragma unroll
       for (int j = 0; j < C::LOADS_W; ++j) {
         const int gi = wv * C::LOADS_W + j;  // wave-uniform
@@ -121,24 +138,27 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
     };
 
     v4f acc[C::FM][C::FN], accx[C::FM][C::FN];
-    double acc64[C::FM][C::FN][4];
+    double acc64[ACC64 ? C::FM : 1][ACC64 ? C::FN : 1][4];
 #pragma unroll
     for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
-      for (int ni = 0; ni < C::FN; ++ni)
+      for (int ni = 0; ni < C::FN; ++ni) {
+        acc[mi][ni] = (v4f)(0.0f);
+        accx[mi][ni] = (v4f)(0.0f);
+        if (ACC64)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc64[mi][ni][q] = 0.0;
+          for (int q = 0; q < 4; ++q) acc64[ACC64 ? mi : 0][ACC64 ? ni : 0][q] = 0.0;
+      }
 
 #pragma unroll
-    for (int ci = 0; ci < H2_STAGES - 1; ++ci)
+    for (int ci = 0; ci < S - 1; ++ci)
       if (ci < nk) issue(ci);
-    // chunk it has landed in LDS for every wave; queue chunk it + H2_STAGES - 1
-    auto sync_issue = [&](int it) {
+    for (int it = 0; it < nk; ++it) {
       // this wave's loads of chunk it have landed once at most
-      // min(H2_STAGES - 2, nk - 1 - it) later chunks remain outstanding
-      const int later = min(H2_STAGES - 2, nk - 1 - it);
-      if (later >= 2) wait_vmcnt<2 * C::LOADS_W>();
-      else if (later == 1) wait_vmcnt<C::LOADS_W>();
+      // min(S - 2, nk - 1 - it) later chunks remain outstanding
+      const int later = min(S - 2, nk - 1 - it);
+      if (S > 3 && later >= 2) wait_vmcnt<(S > 3 ? 2 : 0) * C::LOADS_W>();
+      else if (S > 2 && later >= 1) wait_vmcnt<(S > 2 ? 1 : 0) * C::LOADS_W>();
       else wait_vmcnt<0>();
       // every wave's loads of chunk it landed and every wave is done with chunk
       // it - 1.  A bare s_barrier: the fence of __syncthreads would also wait for
@@ -147,12 +167,17 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
       __asm__ __volatile__("" ::: "memory");
       __builtin_amdgcn_s_barrier();
       __asm__ __volatile__("" ::: "memory");
-      if (it + H2_STAGES - 1 < nk) issue(it + H2_STAGES - 1);
-    };
-    // acc += A_hi B_hi and accx += A_hi B_lo + A_lo B_hi over chunk it; the lo
-    // planes carry 2^11 (x - hi), so accx is weighted by 2^-11 at the flush
-    auto mma = [&](int it) {
-      const unsigned char* cur = lds + (it % H2_STAGES) * C::STAGE;
+      if (it + S - 1 < nk) issue(it + S - 1);
+      if (ACC64 && it % seg == 0 && it > 0) {
+#pragma unroll
+        for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < C::FN; ++ni) {
+            acc[mi][ni] = (v4f)(0.0f);
+            accx[mi][ni] = (v4f)(0.0f);
+          }
+      }
+      const unsigned char* cur = lds + (it % S) * C::STAGE;
       half8 ah[C::FM], al[C::FM], bh[C::FN], bl[C::FN];
 #pragma unroll
       for (int f = 0; f < C::FM; ++f) {
@@ -166,7 +191,8 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
         bh[f] = *(const half8*)(cur + 2 * C::PA + o);
         bl[f] = *(const half8*)(cur + 2 * C::PA + C::PB + o);
       }
-      // three passes over the fragments: no back-to-back dependent MFMAs
+      // acc += A_hi B_hi, accx += A_lo B_hi + A_hi B_lo; three passes over the
+      // fragments, so no MFMA waits on the one before it
 #pragma unroll
       for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
@@ -185,29 +211,16 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
         for (int ni = 0; ni < C::FN; ++ni)
           accx[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bl[ni], accx[mi][ni], 0,
                                                                 0, 0);
-    };
-    auto flush = [&]() {
-#pragma unroll
-      for (int mi = 0; mi < C::FM; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < C::FN; ++ni)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            acc64[mi][ni][q] += (double)acc[mi][ni][q] + H2_LO * (double)accx[mi][ni][q];
-    };
-    for (int it = 0; it < nk; ++it) {
-      sync_issue(it);
-      if (it % seg == 0) {
+      if (ACC64 && ((it + 1) % seg == 0 || it + 1 == nk)) {
 #pragma unroll
         for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
-          for (int ni = 0; ni < C::FN; ++ni) {
-            acc[mi][ni] = (v4f)(0.0f);
-            accx[mi][ni] = (v4f)(0.0f);
-          }
+          for (int ni = 0; ni < C::FN; ++ni)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              acc64[ACC64 ? mi : 0][ACC64 ? ni : 0][q] +=
+                  (double)acc[mi][ni][q] + H2_LO * (double)accx[mi][ni][q];
       }
-      mma(it);
-      if ((it + 1) % seg == 0 || it + 1 == nk) flush();
     }
     // all waves done reading the ring before the next tile's loads overwrite it
     __syncthreads();
@@ -222,7 +235,9 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
 #pragma unroll
         for (int ni = 0; ni < C::FN; ++ni) {
           const int64_t col = col0 + wx * C::WN + ni * 16 + l16;
-          const float v = (float)(acc64[mi][ni][q] * f);
+          const double sum = ACC64 ? acc64[ACC64 ? mi : 0][ACC64 ? ni : 0][q]
+                                   : (double)acc[mi][ni][q] + H2_LO * (double)accx[mi][ni][q];
+          const float v = (float)(sum * f);
           g.C[row * g.ldc + col] = v;
           if (g.Ch) {
             const float x = v * g.c_scale;
@@ -235,8 +250,9 @@ __global__ __launch_bounds__(256, 1) void gemm_h2_kernel(H2Args g) {
   }
 }
 
-// Snake-schedule cost model: chunk counts per worker, CU time = the sum over
-// the workers it hosts (one workgroup per CU at a time), per-tile fixed cost.
+// Cost model of the worker schedule: chunk counts per worker (plus a per-tile
+// fixed cost); a CU's time is the sum over the workers it hosts, worker w on
+// CU w % ncu.
 static double h2_cost(const H2Args& g, int BN, int W) {
   const int MT = g.M / H2_BM, NT = g.N / BN, total = MT * NT;
   const int ncu = num_cus();
@@ -244,9 +260,9 @@ static double h2_cost(const H2Args& g, int BN, int W) {
   const double per_chunk = BN == 128 ? 1.0 : 0.55;  // MFMA + LDS time of one K chunk
   for (int w = 0; w < W; ++w) {
     double load = 0.0;
-    for (int j = 0;; ++j) {
-      const int t = (j & 1) ? (j + 1) * W - 1 - w : j * W + w;
-      if (t >= total) break;
+    for (int j = 0; j * W < total; ++j) {
+      const int t = h2_tile_of(j, w, W);
+      if (t >= total) continue;
       int tm, tn;
       decode_tile(t, g.flags, MT, NT, tm, tn);
       int kb = 0, ke = g.K;
@@ -263,22 +279,23 @@ struct H2Plan {
   int bn, workers;
 };
 
-static H2Plan h2_plan(const H2Args& g) {
+static H2Plan h2_plan(const H2Args& g, bool acc64) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int, int>, H2Plan> cache;
-  const auto key = std::make_tuple(g.M, g.N, g.K, g.flags);
+  static std::map<std::tuple<int, int, int, int, int>, H2Plan> cache;
+  const auto key = std::make_tuple(g.M, g.N, g.K, g.flags, (int)acc64);
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
   }
   const int ncu = num_cus();
-  H2Plan best{128, 1};
+  const int slots = acc64 ? ncu : 2 * ncu;  // resident workgroups
+  H2Plan best{64, 1};
   double bt = 1e300;
-  for (int bn : {128, 64}) {
+  for (int bn : {64, 128}) {
     if (g.N % bn) continue;
     const int total = (g.M / H2_BM) * (g.N / bn);
-    for (int c : {ncu, 3 * ncu / 4, ncu / 2}) {
+    for (int c : {slots, slots / 2, 3 * slots / 8, slots / 4}) {
       const int W = std::max(1, std::min(c, total));
       const double t = h2_cost(g, bn, W);
       if (t < bt) {
@@ -296,17 +313,21 @@ int gemm_h2(const H2Args& a, hipStream_t s) {
   const uintptr_t al16 = (uintptr_t)a.Ah | (uintptr_t)a.Al | (uintptr_t)a.Bh | (uintptr_t)a.Bl;
   if (a.M % H2_BM || a.N % 64 || a.K % H2_BK || a.M <= 0 || a.N <= 0 || a.K <= 0 ||
       a.lda % 8 || a.ldb % 8 || a.lda < a.K || a.ldb < a.K || a.ldc < a.N || !a.Ah || !a.Al ||
-      !a.Bh || !a.Bl || !a.C || (a.Ch && !a.Cl) || (al16 & 15) ||
+      !a.Bh || !a.Bl || !a.C || (a.Ch && !a.Cl) || (al16 & 15) || a.seg_chunks < 0 ||
       (a.flags & ~(BTA_GEMM_B_LOWER | BTA_GEMM_B_UPPER))) {
     set_error("gemm_h2: M%%128, N%%64, K%%32, lda/ldb%%8 must be 0, operand planes 16-byte "
               "aligned, K > 0; flags B_LOWER/B_UPPER only");
     return BTA_ERR_INVALID;
   }
-  const H2Plan p = h2_plan(a);
-  if (p.bn == 128)
-    hipLaunchKernelGGL(gemm_h2_kernel<128>, dim3(p.workers), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(gemm_h2_kernel<64>, dim3(p.workers), dim3(256), 0, s, a);
+  const bool acc64 = a.seg_chunks > 0;
+  const H2Plan p = h2_plan(a, acc64);
+  if (p.bn == 128) {
+    if (acc64) hipLaunchKernelGGL((gemm_h2_kernel<128, true>), dim3(p.workers), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((gemm_h2_kernel<128, false>), dim3(p.workers), dim3(256), 0, s, a);
+  } else {
+    if (acc64) hipLaunchKernelGGL((gemm_h2_kernel<64, true>), dim3(p.workers), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((gemm_h2_kernel<64, false>), dim3(p.workers), dim3(256), 0, s, a);
+  }
   BTA_LAUNCH_CHECK("gemm_h2_kernel");
   return BTA_OK;
 }

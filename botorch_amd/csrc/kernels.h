@@ -44,7 +44,8 @@ struct H2Args {
   int64_t lda, ldb, ldc;
   int M, N, K, flags;       // flags: BTA_GEMM_B_LOWER / BTA_GEMM_B_UPPER (N x K view)
   double alpha;
-  int seg_chunks;           // K chunks of 32 per FP64 flush (0: 8, i.e. 256 K)
+  int seg_chunks;           // 0: FP32 sums over all of K; > 0: FP64 sums of FP32
+                            // segments of seg_chunks x 32 K
 };
 int gemm_h2(const H2Args& a, hipStream_t s);
 int split_f16(bool f64, const void* X, int64_t ldx, int64_t rows, int64_t cols,

@@ -129,15 +129,14 @@ int bta_split_f16(int32_t src_f64, const void* X, int64_t ldx, int64_t rows, int
                   void* hi, void* lo, int64_t ldo, bta_stream_t stream);
 
 /* C = alpha (A_hi + 2^-11 A_lo)(B_hi + 2^-11 B_lo)^T / (a_scale a_row_scale[m] b_scale)
- * without the lo*lo term (planes as bta_split_f16 writes them), on FP16 MFMA
- * with FP64 sums of FP32 K segments.
+ * without the lo*lo term (planes as bta_split_f16 writes them), on FP16 MFMA.
  * A planes M x K (stride lda), B planes N x K (stride ldb), FP16; C float M x N
  * (stride ldc).  When c_hi is not NULL the planes of c_scale * C are also
  * written (stride ldc).  flags: BTA_GEMM_B_LOWER / BTA_GEMM_B_UPPER.  M
  * multiple of 128, N of 64, K of 32, lda/ldb of 8; a_row_scale may be NULL.
- * k_segment: K per FP32 accumulation segment before the FP64 add (a multiple
- * of 32; 0 selects 256).  No counterpart in the reference: the contraction
- * kernel of BTA_PREC_MIXED. */
+ * k_segment: 0 accumulates in FP32 over all of K; otherwise FP32 segments of
+ * k_segment (a multiple of 32) are added into FP64 sums.  No counterpart in the
+ * reference: the contraction kernel of BTA_PREC_MIXED. */
 int bta_gemm_f16x3(int32_t flags, int64_t M, int64_t N, int64_t K, double alpha,
                    const void* a_hi, const void* a_lo, int64_t lda, const float* a_row_scale,
                    float a_scale, const void* b_hi, const void* b_lo, int64_t ldb,
