@@ -59,6 +59,68 @@ __device__ __forceinline__ double kern_grad_factor(int kind, double r2, double s
   return s * e;
 }
 
+// e^x for x <= 0 (the kernels' arguments): Cody-Waite reduction x = k ln2 + f
+// with ln2 split in two (|f| <= ln2/2), then the degree-12 Taylor polynomial
+// of e^f (remainder < 2e-16 relative) and an exact power-of-two scaling.
+// Below -745 the result underflows to 0; NaN propagates.
+__host__ __device__ __forceinline__ double exp_nonpos(double x) {
+  const bool under = x < -745.2;
+  x = under ? 0.0 : x;
+  const double kf = rint(x * 1.4426950408889634074);
+  double f = fma(kf, -6.93147180369123816490e-01, x);
+  f = fma(kf, -1.90821492927058770002e-10, f);
+  double p = 1.0 / 479001600.0;  // 1/12!
+  p = fma(p, f, 1.0 / 39916800.0);
+  p = fma(p, f, 1.0 / 3628800.0);
+  p = fma(p, f, 1.0 / 362880.0);
+  p = fma(p, f, 1.0 / 40320.0);
+  p = fma(p, f, 1.0 / 5040.0);
+  p = fma(p, f, 1.0 / 720.0);
+  p = fma(p, f, 1.0 / 120.0);
+  p = fma(p, f, 1.0 / 24.0);
+  p = fma(p, f, 1.0 / 6.0);
+  p = fma(p, f, 0.5);
+  p = fma(p, f, 1.0);
+  p = fma(p, f, 1.0);
+  return under ? 0.0 : ldexp(p, (int)kf);
+}
+
+// sqrt(r2) for r2 >= 1e-30: v_rsq_f64 seed, one Goldschmidt step for the
+// root and half-reciprocal pair, two Newton corrections of the root.
+__device__ __forceinline__ double sqrt_pos(double r2) {
+  const double y = __builtin_amdgcn_rsq(r2);
+  double r = r2 * y, h = 0.5 * y;
+  const double e = fma(-r, h, 0.5);
+  r = fma(r, e, r);
+  h = fma(h, e, h);
+  double d = fma(-r, r, r2);
+  r = fma(d, h, r);
+  d = fma(-r, r, r2);
+  return fma(d, h, r);
+}
+
+// Compile-time kernel kind: one code path per kernel instantiation.  The
+// Matern distance is clamped at 1e-30 as gpytorch's MaternKernel does.
+template <int KIND>
+__device__ __forceinline__ double kern_value_k(double r2, double s) {
+  if (KIND == BTA_KERNEL_MATERN52) {
+    const double r = sqrt_pos(fmax(r2, 1e-30));
+    const double sr = 2.2360679774997896964 * r;
+    return s * (1.0 + sr + (5.0 / 3.0) * r2) * exp_nonpos(-sr);
+  }
+  return s * exp_nonpos(-0.5 * r2);
+}
+
+template <int KIND>
+__device__ __forceinline__ double kern_grad_factor_k(double r2, double s) {
+  if (KIND == BTA_KERNEL_MATERN52) {
+    const double r = sqrt_pos(fmax(r2, 1e-30));
+    const double sr = 2.2360679774997896964 * r;
+    return s * (5.0 / 3.0) * (1.0 + sr) * exp_nonpos(-sr);
+  }
+  return s * exp_nonpos(-0.5 * r2);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -72,7 +72,7 @@ struct alignas(16) QsKStage {
 };
 
 // mode: 0 = FP64 K, 1 = FP32 K, 2 = FP16 hi/lo planes of K * kscale
-template <int KD, int MODE>
+template <int KD, int MODE, int KIND>
 __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
   typedef typename std::conditional<MODE == 0, double,
                                     typename std::conditional<MODE == 1, float,
@@ -113,11 +113,12 @@ __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
     for (int q = 0; q < 4; ++q) {
       const int cl = cb * 16 + g + 4 * q;
       const double r2 = fmax(sm.tn2[cl] + xn2 - 2.0 * cr[q], 0.0);
-      double k = 0.0;
-      if (c0 + cl < a.n && rl < a.R) k = kern_value(a.kind, r2, a.s);
+      double k = kern_value_k<KIND>(r2, a.s);
+      k = (c0 + cl < a.n && rl < a.R) ? k : 0.0;
       kv[cb][q] = k;
       if (MODE == 2) {
-        const float x = (float)(k * a.kscale);
+        // kscale is a power of two: scaling after the FP32 rounding is exact
+        const float x = (float)k * a.kscale;
         const _Float16 h = (_Float16)x;
         ks[0].k[wv * 16 + l16][cl] = h;
         ks[MODE == 2 ? 1 : 0].k[wv * 16 + l16][cl] = (_Float16)((x - (float)h) * 2048.0f);
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
   }
 }
 
-template <int KD, typename TU>
+template <int KD, typename TU, int KIND>
 __global__ __launch_bounds__(256) void qstep_grad_kernel(QsGradArgs a) {
   constexpr int NB = (4 * KD + 1 + 15) / 16;  // 16-column blocks of [T | 1]
   __shared__ QsT<KD> sm;
@@ -211,8 +212,9 @@ __global__ __launch_bounds__(256) void qstep_grad_kernel(QsGradArgs a) {
     for (int s = 0; s < ku; ++s) {
       const int64_t u = kr0 + 4 * s + g;
       const double ua = (u < kr1 && ca < a.n) ? (double)U[u * a.ldu + ca] : 0.0;
-      const double mb = (rvalid && u < kr1 && u - u % q == rblk) ? a.gq[rl * q + (u - rblk)]
-                                                                   : 0.0;
+      // u in this lane's restart [rblk, rblk + q) (rows beyond R never are)
+      const bool same = rvalid && u >= rblk && u < rblk + q;
+      const double mb = same ? a.gq[rl * q + (u - rblk)] : 0.0;
       ga = mfma64(ua, mb, ga);
     }
     // w^T[c][r] = -G h(r^2); then [T | 1]-contraction over c
@@ -220,8 +222,8 @@ __global__ __launch_bounds__(256) void qstep_grad_kernel(QsGradArgs a) {
     for (int qq = 0; qq < 4; ++qq) {
       const int cl = cb * 16 + g + 4 * qq;
       const double r2 = fmax(sm.tn2[cl] + xn2 - 2.0 * cr[qq], 0.0);
-      double w = 0.0;
-      if (c0 + cl < a.n && rvalid) w = -ga[qq] * kern_grad_factor(a.kind, r2, a.s, nullptr);
+      double w = -ga[qq] * kern_grad_factor_k<KIND>(r2, a.s);
+      w = (c0 + cl < a.n && rvalid) ? w : 0.0;
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
         const int j = nb * 16 + l16;
@@ -280,14 +282,22 @@ static int qs_check(const char* what, int d, int ne, int64_t n, int64_t n_pad, i
   return BTA_OK;
 }
 
+template <int KD, int KIND>
+static void covar_launch_k(const QsCovarArgs& a, int mode, dim3 grid, hipStream_t st) {
+  if (mode == 0)
+    hipLaunchKernelGGL((qstep_covar_kernel<KD, 0, KIND>), grid, dim3(256), 0, st, a);
+  else if (mode == 1)
+    hipLaunchKernelGGL((qstep_covar_kernel<KD, 1, KIND>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((qstep_covar_kernel<KD, 2, KIND>), grid, dim3(256), 0, st, a);
+}
+
 template <int KD>
 static void covar_launch(const QsCovarArgs& a, int mode, dim3 grid, hipStream_t st) {
-  if (mode == 0)
-    hipLaunchKernelGGL((qstep_covar_kernel<KD, 0>), grid, dim3(256), 0, st, a);
-  else if (mode == 1)
-    hipLaunchKernelGGL((qstep_covar_kernel<KD, 1>), grid, dim3(256), 0, st, a);
+  if (a.kind == BTA_KERNEL_MATERN52)
+    covar_launch_k<KD, BTA_KERNEL_MATERN52>(a, mode, grid, st);
   else
-    hipLaunchKernelGGL((qstep_covar_kernel<KD, 2>), grid, dim3(256), 0, st, a);
+    covar_launch_k<KD, BTA_KERNEL_RBF>(a, mode, grid, st);
 }
 
 int qstep_covar(const QsCovarArgs& a, int mode, hipStream_t st) {
@@ -312,12 +322,20 @@ int qstep_covar(const QsCovarArgs& a, int mode, hipStream_t st) {
   return BTA_OK;
 }
 
+template <int KD, int KIND>
+static void grad_launch_k(const QsGradArgs& a, bool u_f32, dim3 grid, hipStream_t st) {
+  if (u_f32)
+    hipLaunchKernelGGL((qstep_grad_kernel<KD, float, KIND>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((qstep_grad_kernel<KD, double, KIND>), grid, dim3(256), 0, st, a);
+}
+
 template <int KD>
 static void grad_launch(const QsGradArgs& a, bool u_f32, dim3 grid, hipStream_t st) {
-  if (u_f32)
-    hipLaunchKernelGGL((qstep_grad_kernel<KD, float>), grid, dim3(256), 0, st, a);
+  if (a.kind == BTA_KERNEL_MATERN52)
+    grad_launch_k<KD, BTA_KERNEL_MATERN52>(a, u_f32, grid, st);
   else
-    hipLaunchKernelGGL((qstep_grad_kernel<KD, double>), grid, dim3(256), 0, st, a);
+    grad_launch_k<KD, BTA_KERNEL_RBF>(a, u_f32, grid, st);
 }
 
 int qstep_grad(const QsGradArgs& a, bool u_f32, const double* extra, const double* ls,
