@@ -16,7 +16,10 @@
 // over all pairs of the level, so the whole inverse is ~4 log2(n/64) launches.
 #include "bta_common.h"
 #include "gemm.h"
+#include "gemm_d.h"
 #include "kernels.h"
+
+#include <algorithm>
 
 namespace bta {
 
@@ -55,50 +58,451 @@ __device__ int chol_lds(double* Ls, int n, int* nan_seen) {
   return fail;
 }
 
-// Inverse of the lower-triangular n x n (n <= 64) LDS matrix: thread j < n
-// produces column j of L^-1 by forward substitution and writes it to out.
-__device__ void trinv_lds_store(const double* Ls, int n, double* out, int64_t ldo) {
-  const int j = threadIdx.x;
-  if (j >= LEAF) return;
-  double x[LEAF];
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const int2 p = __builtin_bit_cast(int2, v);
+  int2 r;
+  r.x = __builtin_amdgcn_readlane(p.x, l);
+  r.y = __builtin_amdgcn_readlane(p.y, l);
+  return __builtin_bit_cast(double, r);
+}
+
+// v from the lane given by the quad permutation CTRL (DPP, no LDS traffic)
+template <int CTRL>
+__device__ __forceinline__ double quad_perm_d(double v) {
+  const int2 p = __builtin_bit_cast(int2, v);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, false);
+  r.y = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, r);
+}
+constexpr int QP_XOR1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int QP_XOR2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+
+// Waits for this wave's outstanding LDS operations.  Lanes of one wave that
+// exchange data through LDS without a barrier need it between the write and
+// the reads (and before overwriting what other lanes still read): with a
+// second workgroup on the CU, a wave's LDS accesses are not reliably
+// performed in issue order.
+__device__ __forceinline__ void lds_wait() { __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// One 16 x 16 x 16 FP64 MFMA product on LDS operands X (pitch PX), Y (pitch
+// PY): acc += sgn * X[xr + i][xk + k] B[k][n] with B[k][n] = Y[yr + n][yk + k]
+// (BT, a product with Y^T) or Y[yk + k][yr + n] (!BT).  Accumulator layout:
+// acc[q] = C[g + 4q][l16].
+template <bool BT, int PX = LP, int PY = LP>
+__device__ __forceinline__ void mm16(const double* X, int xr, int xk, const double* Y, int yr,
+                                     int yk, double sgn, v4d& acc) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int i = 0; i < LEAF; ++i) {
-    double acc = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 0; k < i; ++k) acc -= Ls[i * LP + k] * x[k];
-    x[i] = (i < n) ? acc / Ls[i * LP + i] : 0.0;
-  }
-  if (j < n) {
-#pragma unroll
-    for (int i = 0; i < LEAF; ++i)
-      if (i < n) out[(int64_t)i * ldo + j] = x[i];
+  for (int s = 0; s < 4; ++s) {
+    const double av = sgn * X[(xr + l16) * PX + xk + 4 * s + g];
+    const double bv = BT ? Y[(yr + l16) * PY + yk + 4 * s + g] : Y[(yk + 4 * s + g) * PY + yr + l16];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
   }
 }
 
-// info[0]: first failing column + 1 (0 = ok); info[1]: NaN pivot seen.
-__global__ __launch_bounds__(256) void potrf_leaf_kernel(double* A, int64_t lda, double* W,
-                                                         int64_t ldw, int j0, int* info) {
-  __shared__ double Ls[LEAF * LP];
-  __shared__ int nan_seen;
-  if (info[0] != 0) return;  // an earlier panel already failed
-  if (threadIdx.x == 0) nan_seen = 0;
-  for (int e = threadIdx.x; e < LEAF * LEAF; e += 256) {
-    int i = e / LEAF, j = e % LEAF;
-    Ls[i * LP + j] = j <= i ? A[(int64_t)i * lda + j] : 0.0;
+template <int P = LP>
+__device__ __forceinline__ v4d ld16(const double* X, int r0, int c0) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  v4d v;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = X[(r0 + g + 4 * q) * P + c0 + l16];
+  return v;
+}
+
+template <int P = LP>
+__device__ __forceinline__ void st16(double* X, int r0, int c0, const v4d& v) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) X[(r0 + g + 4 * q) * P + c0 + l16] = v[q];
+}
+
+// 64 x 64 global block -> LDS (pitch LP): all 16 loads of a thread are issued
+// before its LDS stores.
+__device__ __forceinline__ void stage64(const double* __restrict__ src, int64_t ld, double* dst) {
+  const int tid = threadIdx.x;
+  double v[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int e = tid + 256 * t;
+    v[t] = src[(int64_t)(e >> 6) * ld + (e & 63)];
   }
-  const int fail = chol_lds(Ls, LEAF, &nan_seen);
-  if (fail >= 0) {
-    if (threadIdx.x == 0) {
-      info[0] = j0 + fail + 1;
-      info[1] = nan_seen;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int e = tid + 256 * t;
+    dst[(e >> 6) * LP + (e & 63)] = v[t];
+  }
+}
+
+// Work the other three waves do while wave 0 factors a panel.
+struct NoBackground {
+  __device__ void operator()(int) const {}
+};
+
+// Cholesky of the 64 x 64 block in S (pitch LP), right-looking over four
+// 16-column panels.  Wave 0 factors a panel (all 64 rows, lane i = row i, the
+// panel's 16 entries in registers): the pivot of column c comes from lane c
+// by readlane, 1/sqrt(pivot) from v_rsq_f64 with two Newton steps (sqrt as
+// p * rsqrt with one correction).  The next column's multiplier also goes by
+// readlane, which keeps the pivot chain short; the later columns' come from
+// the column's LDS copy colv in one batch of broadcast reads.  Meanwhile
+// waves 1-3 run bg(p).  Then the four waves apply the panel to the trailing
+// 16 x 16 blocks with FP64 MFMA (rank 16 per block).  No branches on the
+// data: a non-positive or NaN pivot is recorded and replaced by 1.  Leaves L
+// in S (upper part zeroed) and 1 / L_ii in rdiag; *fail_s / *nan_s (LDS) get
+// the first failing column (or -1) and whether its pivot was NaN.
+template <typename BG = NoBackground>
+__device__ __forceinline__ void chol64_blocked(double* S, double* rdiag, double* colv,
+                                               int* fail_s, int* nan_s, BG bg = BG()) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int fail = -1, nanf = 0;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    if (wv == 0) {
+      double a[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) a[m] = S[lane * LP + 16 * p + m];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        const int cc = 16 * p + c;
+        double pv = readlane_d(a[c], cc);
+        const bool bad = !(pv > 0.0);
+        nanf = (fail < 0 && bad) ? (pv != pv) : nanf;
+        fail = (fail < 0 && bad) ? cc : fail;
+        pv = bad ? 1.0 : pv;
+        double y = __builtin_amdgcn_rsq(pv);
+        const double hp = 0.5 * pv;
+        y = fma(y, fma(-hp * y, y, 0.5), y);
+        y = fma(y, fma(-hp * y, y, 0.5), y);
+        double dg = pv * y;
+        dg = fma(fma(-dg, dg, pv), 0.5 * y, dg);
+        const double l = lane == cc ? dg : a[c] * y;
+        a[c] = lane >= cc ? l : 0.0;
+        // rows above the panel and entries right of the diagonal take garbage;
+        // only the lower part is kept
+        if (c + 1 < 16) {
+          lds_wait();  // the previous column's reads of colv are done
+          colv[lane] = l;
+          a[c + 1] = fma(-l, readlane_d(l, cc + 1), a[c + 1]);
+          if (c + 2 < 16) {
+            lds_wait();
+            double cv[16];
+#pragma unroll
+            for (int m = 0; m < 16; m += 2) {
+              const v2d v = *(const v2d*)(colv + 16 * p + m);
+              cv[m] = v.x;
+              cv[m + 1] = v.y;
+            }
+#pragma unroll
+            for (int m = c + 2; m < 16; ++m) a[m] = fma(-l, cv[m], a[m]);
+          }
+          __asm__ __volatile__("" ::: "memory");
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 16; ++m) S[lane * LP + 16 * p + m] = 16 * p + m <= lane ? a[m] : 0.0;
+    } else {
+      bg(p);
+    }
+    __syncthreads();
+    if (p < 3) {
+      // trailing blocks (ib, jb), p < jb <= ib, dealt to the waves
+      const int nt = 4 - p - 1;  // block rows below the panel
+      for (int bi = wv; bi < nt * (nt + 1) / 2; bi += 4) {
+        int ib = 0;
+        while ((ib + 1) * (ib + 2) / 2 <= bi) ++ib;
+        const int jb = bi - ib * (ib + 1) / 2;
+        const int r0 = 16 * (p + 1 + ib), c0 = 16 * (p + 1 + jb);
+        v4d acc = ld16(S, r0, c0);
+        mm16<true>(S, r0, 16 * p, S, c0, 16 * p, -1.0, acc);
+        st16(S, r0, c0, acc);
+      }
+      __syncthreads();
+    }
+  }
+  if (wv == 0) {
+    rdiag[lane] = 1.0 / S[lane * LP + lane];
+    if (lane == 0) {
+      *fail_s = __builtin_amdgcn_readfirstlane(fail);
+      *nan_s = __builtin_amdgcn_readfirstlane(nanf);
+    }
+  }
+  __syncthreads();
+}
+
+// V_b = L_bb^-1 for the four diagonal 16-blocks of L (S, pitch LP, rdiag =
+// 1 / L_ii): wave b forward-substitutes block b, four lanes per column
+// splitting each row's dot product by k mod 4 and combining it by two DPP quad
+// exchanges.  V_b goes to V[(16 b + i) * PV + (DIAG ? 16 b : 0) + j].
+template <int PV, bool DIAG>
+__device__ __forceinline__ void inv16_diag(const double* S, const double* rdiag, double* V) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int jl = lane >> 2, r = lane & 3, b0 = 16 * wv;
+  // this lane's operands L[b0 + i][b0 + 4t + r] (k < i), loaded up front
+  double lv[16][4];
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+#pragma unroll
+    for (int t = 0; 4 * t < i; ++t) {
+      const int k = 4 * t + r;
+      lv[i][t] = k < i ? S[(b0 + i) * LP + b0 + k] : 0.0;
+    }
+  double rd[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) rd[i] = rdiag[b0 + i];
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    double pp = 0.0;
+#pragma unroll
+    for (int t = 0; 4 * t < i; ++t) pp = fma(lv[i][t], x[t], pp);
+    pp += quad_perm_d<QP_XOR1>(pp);
+    pp += quad_perm_d<QP_XOR2>(pp);
+    const double xi = ((i == jl ? 1.0 : 0.0) - pp) * rd[i];
+    if (r == (i & 3)) x[i >> 2] = xi;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) V[(b0 + 4 * t + r) * PV + (DIAG ? b0 : 0) + jl] = x[t];
+}
+
+// W = L^-1 (pitch LP) for the lower-triangular L in S, with rdiag = 1 / L_ii:
+// the diagonal 16-blocks by inv16_diag, the off-diagonal blocks by recursive
+// doubling on FP64 MFMA:  W10 = -V1 L10 V0,  W32 = -V3 L32 V2, then
+// W[23][01] = -W[23][23] (L[23][01] W[01][01]).  All 256 threads call it.
+__device__ __forceinline__ void trinv64_blocked(const double* S, const double* rdiag, double* Wl) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  inv16_diag<LP, true>(S, rdiag, Wl);
+  {
+    // zero this wave's block row right of its diagonal block
+    const int b0 = 16 * wv;
+    for (int e = lane; e < 16 * 48; e += 64) {
+      const int rr = e / 48, cc = e % 48;
+      const int col = b0 + 16 + cc;
+      if (col < LEAF) Wl[(b0 + rr) * LP + col] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (wv < 2) {  // W10 (wave 0), W32 (wave 1)
+    const int i = 2 * wv + 1, j = 2 * wv;
+    v4d t = (v4d)(0.0);
+    mm16<false>(S, 16 * i, 16 * j, Wl, 16 * j, 16 * j, 1.0, t);  // L_ij V_j
+    st16(Wl, 16 * i, 16 * j, t);
+    lds_wait();
+    v4d w = (v4d)(0.0);
+    mm16<false>(Wl, 16 * i, 16 * i, Wl, 16 * j, 16 * i, -1.0, w);  // -V_i T
+    st16(Wl, 16 * i, 16 * j, w);
+  }
+  __syncthreads();
+  // T2_ij = sum_k L_ik W_kj, i in {2,3}, j, k in {0,1}: wave -> (i, j)
+  const int i2 = 2 + (wv >> 1), j2 = wv & 1;
+  v4d t2 = (v4d)(0.0);
+  mm16<false>(S, 16 * i2, 16, Wl, 16 * j2, 16, 1.0, t2);            // L_i1 W_1j
+  if (j2 == 0) mm16<false>(S, 16 * i2, 0, Wl, 0, 0, 1.0, t2);        // L_i0 W_00
+  st16(Wl, 16 * i2, 16 * j2, t2);
+  __syncthreads();
+  // W_ij = -sum_{k in {2,3}, k <= i} W_ik T2_kj
+  v4d w2 = (v4d)(0.0);
+  mm16<false>(Wl, 16 * i2, 32, Wl, 16 * j2, 32, -1.0, w2);          // W_i2 T2_2j
+  if (i2 == 3) mm16<false>(Wl, 48, 48, Wl, 16 * j2, 48, -1.0, w2);   // W_33 T2_3j
+  __syncthreads();
+  st16(Wl, 16 * i2, 16 * j2, w2);
+  __syncthreads();
+}
+
+// One step of the blocked factorisation, as one launch.  Workgroups
+// [0, nstrip) are the strip of block column j: workgroup s owns row block
+// b = j + s.  Each applies column j-1's update to the diagonal block (all of
+// them, redundantly, so none waits for another) and to its own block, and
+// factors the diagonal block (chol64_blocked).  Workgroup 0 (b = j) writes
+// L_jj; the others solve their block, L_bj L_jj^T = A_bj - L_b,j-1 L_j,j-1^T,
+// by blocked forward substitution: wave w takes rows [16w, 16w + 16) and the
+// four 16-column blocks in turn, X_k = (T_k - sum_{m<k} X_m L_km^T) V_k^T
+// with V_k = L_kk^-1 from inv16_diag, so no wave waits for another.  The
+// remaining workgroups apply column j-1's update to the trailing lower tiles
+// right of column j (the strip covers column j): A_bc -= L_b,j-1 L_c,j-1^T on
+// 128 x 128 FP64 MFMA tiles.
+struct StepArgs {
+  double* A;
+  int64_t lda;
+  int* info;
+  int j, NB, nstrip;
+};
+
+constexpr int VP = 17;  // pitch of the 16 x 16 diagonal inverses (64 x 16 array)
+constexpr int STEP_LDS_STRIP = (2 * LEAF * LP + LEAF * VP + 2 * LEAF) * 8 + 16;
+constexpr int STEP_LDS = STEP_LDS_STRIP > DCfg<128>::STAGES * DCfg<128>::STAGE
+                             ? STEP_LDS_STRIP
+                             : DCfg<128>::STAGES * DCfg<128>::STAGE;
+
+__device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* smem) {
+  double* S = (double*)smem;     // diagonal block -> L_jj
+  double* Bf = S + LEAF * LP;    // L_j,j-1 staging, then this block's T -> X
+  double* Vs = Bf + LEAF * LP;   // V_k, k = 0..3 (rows 16k.., pitch VP)
+  double* rdiag = Vs + LEAF * VP;
+  double* colv = rdiag + LEAF;
+  int* flags = (int*)(colv + LEAF);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, l16 = lane & 15, g = lane >> 4;
+  const int j = a.j, b = j + blockIdx.x;
+  const bool own = b != j;  // workgroup-uniform
+  const int64_t lda = a.lda;
+  double* Ajj = a.A + (int64_t)j * LEAF * lda + (int64_t)j * LEAF;
+  double* Abj = a.A + (int64_t)b * LEAF * lda + (int64_t)j * LEAF;
+  const double* Q = Abj - LEAF;  // L_b,j-1 (j > 0)
+  // D blocks (ib, jb), jb <= ib (10 of them), dealt round-robin over the waves
+  v4d dacc[3];
+  int dib[3], djb[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int bi = wv + 4 * u;
+    int ib = 0;
+    while ((ib + 1) * (ib + 2) / 2 <= bi) ++ib;
+    dib[u] = ib;
+    djb[u] = bi - ib * (ib + 1) / 2;
+    if (bi < 10)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        dacc[u][q] = Ajj[(int64_t)(16 * dib[u] + g + 4 * q) * lda + 16 * djb[u] + l16];
+  }
+  v4d tacc[4];
+  if (own)
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        tacc[nb][q] = Abj[(int64_t)(16 * wv + g + 4 * q) * lda + 16 * nb + l16];
+  if (j > 0) {
+    stage64(Ajj - LEAF, lda, Bf);  // L_j,j-1
+    __syncthreads();
+    double qv[16];
+    if (own)
+#pragma unroll
+      for (int s = 0; s < 16; ++s) qv[s] = -Q[(int64_t)(16 * wv + l16) * lda + 4 * s + g];
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+      if (wv + 4 * u < 10)
+#pragma unroll
+        for (int s = 0; s < 16; ++s)
+          dacc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              -Bf[(16 * dib[u] + l16) * LP + 4 * s + g], Bf[(16 * djb[u] + l16) * LP + 4 * s + g],
+              dacc[u], 0, 0, 0);
+    if (own)
+#pragma unroll
+      for (int s = 0; s < 16; ++s)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+          tacc[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              qv[s], Bf[(16 * nb + l16) * LP + 4 * s + g], tacc[nb], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (wv + 4 * u < 10) st16(S, 16 * dib[u], 16 * djb[u], dacc[u]);
+  if (own)
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) st16(Bf, 16 * wv, 16 * nb, tacc[nb]);
+  __syncthreads();
+  chol64_blocked(S, rdiag, colv, flags, flags + 1);
+  if (flags[0] >= 0) {
+    if (!own && tid == 0) {
+      a.info[0] = j * LEAF + flags[0] + 1;
+      a.info[1] = flags[1];
     }
     return;
   }
-  for (int e = threadIdx.x; e < LEAF * LEAF; e += 256) {
-    int i = e / LEAF, j = e % LEAF;
-    A[(int64_t)i * lda + j] = j <= i ? Ls[i * LP + j] : 0.0;
+  if (!own) {
+    for (int e = tid; e < LEAF * LEAF; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      if (c <= r) Ajj[(int64_t)r * lda + c] = S[r * LP + c];
+    }
+    return;
   }
-  trinv_lds_store(Ls, LEAF, W, ldw);
+  inv16_diag<VP, false>(S, rdiag, Vs);
+  __syncthreads();
+  const int r0 = 16 * wv;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v4d acc = ld16(Bf, r0, 16 * k);
+#pragma unroll
+    for (int m = 0; m < k; ++m) mm16<true>(Bf, r0, 16 * m, S, 16 * k, 16 * m, -1.0, acc);
+    st16(Bf, r0, 16 * k, acc);
+    lds_wait();
+    v4d x = (v4d)(0.0);
+    mm16<true, LP, VP>(Bf, r0, 16 * k, Vs, 16 * k, 0, 1.0, x);
+    st16(Bf, r0, 16 * k, x);
+    lds_wait();
+  }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) Abj[(int64_t)(r0 + t) * lda + lane] = Bf[(r0 + t) * LP + lane];
+}
+
+__device__ __forceinline__ void potrf_bulk(const StepArgs& a, unsigned char* smem) {
+  const int j = a.j;
+  const int rem = (a.NB - j - 1) * LEAF;
+  const int64_t lda = a.lda;
+  const double* P = a.A + (int64_t)(j + 1) * LEAF * lda + (int64_t)(j - 1) * LEAF;
+  double* Cm = a.A + (int64_t)(j + 1) * LEAF * lda + (int64_t)(j + 1) * LEAF;
+  const int MT = (rem + D_BM - 1) / D_BM, total = MT * (MT + 1) / 2;
+  const int W = gridDim.x - a.nstrip, w = blockIdx.x - a.nstrip;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
+  for (int jj = 0; jj * W < total; ++jj) {
+    const int t = h2_tile_of(jj, w, W);
+    if (t >= total) continue;
+    int tm, tn;
+    decode_tile(t, BTA_GEMM_C_LOWER, MT, MT, tm, tn);
+    v4d acc[4][4];
+    d_mainloop<128, false, false>(P, lda, P, lda, tm * D_BM, tn * D_BM, rem, rem, 0, LEAF, smem,
+                                  acc);
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
+        if (row >= rem) continue;
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+          const int col = tn * D_BM + wx * 64 + ni * 16 + l16;
+          if (col > row) continue;
+          double* c = Cm + (int64_t)row * lda + col;
+          *c -= acc[mi][ni][q];
+        }
+      }
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void potrf_step_kernel(StepArgs a) {
+  __shared__ __attribute__((aligned(1024))) unsigned char smem[STEP_LDS];
+  __shared__ int stop;
+  if (threadIdx.x == 0) stop = a.info[0];
+  __syncthreads();
+  if (stop != 0) return;  // an earlier step failed
+  if ((int)blockIdx.x < a.nstrip)
+    potrf_strip(a, smem);
+  else
+    potrf_bulk(a, smem);
+}
+
+// W_jj = L_jj^-1 for every 64 x 64 diagonal block of L (one workgroup per
+// block), into Linv's diagonal blocks; the rest of each block row (the upper
+// triangle of L^-1) is zeroed.
+__global__ __launch_bounds__(256) void trinv_diag_kernel(const double* L, int64_t ldl,
+                                                         double* Linv, int64_t ldinv,
+                                                         int64_t n_cols) {
+  __shared__ double S[LEAF * LP], Wl[LEAF * LP], rdiag[LEAF];
+  const int64_t o = (int64_t)blockIdx.x * LEAF;
+  stage64(L + o * ldl + o, ldl, S);
+  __syncthreads();
+  if (threadIdx.x < LEAF) rdiag[threadIdx.x] = 1.0 / S[threadIdx.x * LP + threadIdx.x];
+  __syncthreads();
+  trinv64_blocked(S, rdiag, Wl);
+  for (int e = threadIdx.x; e < LEAF * LEAF; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    Linv[(o + r) * ldinv + o + c] = Wl[r * LP + c];
+  }
+  // the block row right of the diagonal block is zero in L^-1
+  const int64_t rest = (int64_t)n_cols - o - LEAF;
+  for (int r = 0; r < LEAF; ++r)
+    for (int64_t c = threadIdx.x; c < rest; c += 256) Linv[(o + r) * ldinv + o + LEAF + c] = 0.0;
 }
 
 __global__ void fill_int_kernel(int* p, int n, int v) {
@@ -169,9 +573,16 @@ size_t potrf_ws_bytes(int64_t n_pad) {
 }
 
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
-          double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s) {
+          double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
+          int opts) {
+  const bool save = !(opts & POTRF_NO_SAVE);
+  if (!save) max_tries = 0;
   if (n_pad <= 0 || n_pad % LEAF || lda < n_pad || (Linv && ldinv < n_pad)) {
     set_error("potrf: n_pad must be a positive multiple of 64");
+    return BTA_ERR_INVALID;
+  }
+  if (((uintptr_t)A & 15) || (lda & 1)) {
+    set_error("potrf: A must be 16-byte aligned with an even leading dimension");
     return BTA_ERR_INVALID;
   }
   if (!ws || ws_bytes < potrf_ws_bytes(n_pad)) {
@@ -182,7 +593,8 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
   double* wblk = saved + n_pad * n_pad;
   int* info = (int*)(wblk + LEAF * LEAF);
   if (jitter_used) *jitter_used = 0.0;
-  {
+  const int NB = (int)(n_pad / LEAF);
+  if (save) {
     dim3 g((unsigned)((n_pad + 255) / 256 > 64 ? 64 : (n_pad + 255) / 256), (unsigned)n_pad);
     hipLaunchKernelGGL(copy_lower_kernel, g, dim3(256), 0, s, A, lda, saved, n_pad, n_pad);
     BTA_LAUNCH_CHECK("copy_lower_kernel");
@@ -200,30 +612,37 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
     }
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, info, 2, 0);
     BTA_LAUNCH_CHECK("fill_int_kernel");
-    for (int64_t j0 = 0; j0 < n_pad; j0 += LEAF) {
-      double* Ajj = A + j0 * lda + j0;
-      double* W = Linv ? Linv + j0 * ldinv + j0 : wblk;
-      const int64_t ldw = Linv ? ldinv : LEAF;
-      hipLaunchKernelGGL(potrf_leaf_kernel, dim3(1), dim3(256), 0, s, Ajj, lda, W, ldw, (int)j0,
-                         info);
-      BTA_LAUNCH_CHECK("potrf_leaf_kernel");
-      const int64_t rem = n_pad - j0 - LEAF;
-      if (rem <= 0) break;
-      double* P = A + (j0 + LEAF) * lda + j0;
-      int rc = gemm(BTA_GEMM_B_LOWER, rem, LEAF, LEAF, 1.0, P, lda, W, ldw, 0.0, nullptr, 0, P,
-                    lda, s);
-      if (rc) return rc;
-      double* T = A + (j0 + LEAF) * lda + j0 + LEAF;
-      rc = gemm(BTA_GEMM_C_LOWER, rem, rem, LEAF, -1.0, P, lda, P, lda, 1.0, T, lda, T, lda, s);
-      if (rc) return rc;
+    for (int j = 0; j < NB; ++j) {
+      StepArgs sa;
+      sa.A = A;
+      sa.lda = lda;
+      sa.info = info;
+      sa.j = j;
+      sa.NB = NB;
+      sa.nstrip = NB - j;
+      int bulk = 0;
+      if (j > 0 && NB - j - 1 > 0) {
+        const int MT = (NB - j - 1 + 1) / 2;  // 128-row tiles over the trailing blocks
+        bulk = std::min(MT * (MT + 1) / 2, std::max(2 * num_cus() - sa.nstrip, num_cus()));
+      }
+      hipLaunchKernelGGL(potrf_step_kernel, dim3((unsigned)(sa.nstrip + bulk)), dim3(256), 0, s,
+                         sa);
+      BTA_LAUNCH_CHECK("potrf_step_kernel");
     }
     int hinfo[2];
     BTA_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s));
     BTA_HIP(hipStreamSynchronize(s));
     if (hinfo[0] == 0) {
       dim3 g((unsigned)((n_pad + 255) / 256 > 64 ? 64 : (n_pad + 255) / 256), (unsigned)n_pad);
-      hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, A, lda, n_pad);
-      BTA_LAUNCH_CHECK("zero_upper_kernel");
+      if (!(opts & POTRF_KEEP_UPPER)) {
+        hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, A, lda, n_pad);
+        BTA_LAUNCH_CHECK("zero_upper_kernel");
+      }
+      if (Linv) {
+        hipLaunchKernelGGL(trinv_diag_kernel, dim3((unsigned)NB), dim3(256), 0, s, A, lda, Linv,
+                           ldinv, n_pad);
+        BTA_LAUNCH_CHECK("trinv_diag_kernel");
+      }
       if (jitter_used) *jitter_used = jit;
       return BTA_OK;
     }
@@ -240,7 +659,7 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
 size_t trtri_ws_bytes(int64_t n_pad) { return (size_t)(n_pad * n_pad / 4 + 64) * sizeof(double); }
 
 int trtri_lower(const double* L, int64_t ldl, double* X, int64_t ldx, int64_t n_pad, void* ws,
-                size_t ws_bytes, hipStream_t s) {
+                size_t ws_bytes, hipStream_t s, int opts) {
   if (n_pad <= 0 || n_pad % LEAF) {
     set_error("trtri: n_pad must be a positive multiple of 64");
     return BTA_ERR_INVALID;
@@ -250,7 +669,7 @@ int trtri_lower(const double* L, int64_t ldl, double* X, int64_t ldx, int64_t n_
     return BTA_ERR_WORKSPACE;
   }
   double* T = (double*)ws;
-  {
+  if (!(opts & TRTRI_UPPER_ZERO)) {
     dim3 g((unsigned)((n_pad + 255) / 256 > 64 ? 64 : (n_pad + 255) / 256), (unsigned)n_pad);
     hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, X, ldx, n_pad);
     BTA_LAUNCH_CHECK("zero_upper_kernel");
@@ -310,7 +729,7 @@ extern "C" int bta_potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, in
     return BTA_ERR_INVALID;
   }
   return bta::potrf(A, n_pad, lda, Linv, ldinv, max_tries, jitter0, jitter_used, ws, ws_bytes,
-                    (hipStream_t)stream);
+                    (hipStream_t)stream, 0);
 }
 
 extern "C" int bta_trtri_lower(const double* L, int64_t ldl, double* Linv, int64_t ldinv,
@@ -319,7 +738,7 @@ extern "C" int bta_trtri_lower(const double* L, int64_t ldl, double* Linv, int64
     bta::set_error("bta_trtri_lower: null pointer");
     return BTA_ERR_INVALID;
   }
-  return bta::trtri_lower(L, ldl, Linv, ldinv, n_pad, ws, ws_bytes, (hipStream_t)stream);
+  return bta::trtri_lower(L, ldl, Linv, ldinv, n_pad, ws, ws_bytes, (hipStream_t)stream, 0);
 }
 
 extern "C" int bta_chol_small_batched(double* A, int32_t n, int64_t lda, int64_t stride,

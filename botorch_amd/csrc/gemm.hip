@@ -17,6 +17,7 @@
 //     product without atomics or a host-side schedule.
 #include "bta_common.h"
 #include "gemm.h"
+#include "gemm_d.h"
 #include "gemm_tile.h"
 #include "kernels.h"
 
@@ -228,6 +229,10 @@ int launch_gemm(const GemmArgs& g, int batch, hipStream_t s) {
     return BTA_ERR_INVALID;
   }
   if (batch == 0) return BTA_OK;
+  if (!(g.flags & BTA_GEMM_F32) && gemm_d_supported(g)) {
+    const int rc = gemm_d(g, batch, s);
+    if (rc >= 0) return rc;
+  }
   const GemmPlan p = plan_gemm(g, batch);
   if (g.flags & BTA_GEMM_F32) {
     if (p.big) return launch_cfg<float, 128, 128, 16>(g, batch, p.W, s);

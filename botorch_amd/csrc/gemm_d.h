@@ -1,0 +1,174 @@
+// FP64 MFMA main loop with direct global -> LDS loads, shared by the FP64
+// GEMM (gemm_d.hip) and the MLL trace kernel.
+#pragma once
+#include "bta_common.h"
+#include "gemm.h"
+#include "gemm_tile.h"
+
+namespace bta {
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+constexpr int D_BM = 128;  // tile rows
+constexpr int D_BK = 8;    // K per chunk (64 bytes of a K-contiguous row)
+
+template <int BN>
+struct DCfg {
+  static constexpr int WN = BN / 2;              // wave tile columns
+  static constexpr int FM = 4, FN = WN / 16;     // 16 x 16 blocks per wave
+  static constexpr int PA = D_BM * D_BK * 8;     // bytes of A per stage
+  static constexpr int PB = BN * D_BK * 8;       // bytes of B per stage
+  static constexpr int STAGE = PA + PB;
+  static constexpr int LA = PA / 1024, LB = PB / 1024;  // 1 KiB wave loads
+  static constexpr int LOADS_W = (LA + LB) / 4;          // per wave per chunk
+  static constexpr int STAGES = 4;
+  static_assert((LA + LB) % 4 == 0, "wave loads must split over 4 waves");
+};
+
+// Slot of 16-byte chunk c of a K-contiguous stage row r is c ^ d_swizzle(r % 16):
+// for a fixed c, rows 0..7 (8..15) fall in distinct slots of a 128-byte line.
+__device__ __forceinline__ int d_swizzle(int r) { return ((r >> 1) & 3) ^ ((r >> 3) & 1); }
+
+// s_waitcnt on the vector-memory counter only (expcnt, lgkmcnt left open)
+template <int N>
+__device__ __forceinline__ void d_wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Tile of position p in round j of W workers: rounds alternate direction, and
+// within a round the second half of the workers takes the tiles in reverse, so
+// workers p and p + W/2 (one CU when two workgroups share it) pair a heavy and
+// a light tile.
+__host__ __device__ inline int h2_tile_of(int j, int p, int W) {
+  const int h = W / 2;
+  int q = (W > 1 && p >= h) ? W - 1 - (p - h) : p;
+  if (j & 1) q = W - 1 - q;
+  return j * W + q;
+}
+
+// acc = op(A)[row0 : row0+128, kb:ke] op(B)[kb:ke, col0 : col0+BN] for the
+// calling wave's 64 x BN/2 part.  A is M x K (lda) or, with AKM, K x M; B is
+// N x K (ldb) or, with BKN, K x N.  Rows/columns at or past M/N read the
+// in-bounds ones 64 earlier.  All 256 threads must call it; it ends with a
+// barrier, so the caller may reuse the LDS afterwards.
+template <int BN, bool AKM, bool BKN>
+__device__ __forceinline__ void d_mainloop(const double* __restrict__ A, int64_t lda,
+                                           const double* __restrict__ B, int64_t ldb, int row0,
+                                           int col0, int M, int N, int kb, int ke,
+                                           unsigned char* lds,
+                                           v4d (&acc)[DCfg<BN>::FM][DCfg<BN>::FN]) {
+  using C = DCfg<BN>;
+  constexpr int S = C::STAGES;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
+  const int lrow = lane >> 2;
+  const int lchunk = (lane & 3) ^ d_swizzle(lrow);
+  const int nk = ke > kb ? (ke - kb) / D_BK : 0;
+
+#pragma unroll
+  for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < C::FN; ++ni) acc[mi][ni] = (v4d)(0.0);
+
+  auto issue = [&](int ci) {
+    const int k0 = kb + ci * D_BK;
+    unsigned char* st = lds + (ci % S) * C::STAGE;
+#pragma unroll
+    for (int j = 0; j < C::LOADS_W; ++j) {
+      const int gi = wv * C::LOADS_W + j;  // wave-uniform
+      const double* src;
+      int off;
+      if (gi < C::LA) {
+        off = gi * 1024;
+        if (AKM) {  // stage rows are k, 128 doubles each: one k row per wave load
+          int m = row0 + lane * 2;
+          if (m >= M) m -= 64;
+          src = A + (int64_t)(k0 + gi) * lda + m;
+        } else {
+          int r = row0 + gi * 16 + lrow;
+          if (r >= M) r -= 64;
+          src = A + (int64_t)r * lda + k0 + lchunk * 2;
+        }
+      } else {
+        const int gb = gi - C::LA;
+        off = C::PA + gb * 1024;
+        if (BKN) {  // stage rows are k, BN doubles each
+          const int e = gb * 128 + lane * 2;
+          const int kr = e / BN;
+          int n = col0 + e % BN;
+          if (n >= N) n -= 64;
+          src = B + (int64_t)(k0 + kr) * ldb + n;
+        } else {
+          int r = col0 + gb * 16 + lrow;
+          if (r >= N) r -= 64;
+          src = B + (int64_t)r * ldb + k0 + lchunk * 2;
+        }
+      }
+      __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(st + off), 16, 0, 0);
+    }
+  };
+
+#pragma unroll
+  for (int ci = 0; ci < S - 1; ++ci)
+    if (ci < nk) issue(ci);
+  const int foff = l16 * 64 + ((gq ^ d_swizzle(l16)) << 4);
+  for (int it = 0; it < nk; ++it) {
+    const int later = min(S - 2, nk - 1 - it);
+    if (later >= 2)
+      d_wait_vmcnt<2 * C::LOADS_W>();
+    else if (later == 1)
+      d_wait_vmcnt<C::LOADS_W>();
+    else
+      d_wait_vmcnt<0>();
+    // every wave's loads of chunk it landed and every wave is done with chunk
+    // it - 1 (whose stage the next issue overwrites).  A bare s_barrier: the
+    // fence of __syncthreads would wait for the loads still in flight.
+    __asm__ __volatile__("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ __volatile__("" ::: "memory");
+    if (it + S - 1 < nk) issue(it + S - 1);
+    const unsigned char* cur = lds + (it % S) * C::STAGE;
+    double af[C::FM][2], bf[C::FN][2];
+#pragma unroll
+    for (int f = 0; f < C::FM; ++f) {
+      if (AKM) {
+        const double* p = (const double*)cur + (wy * 64 + f * 16 + l16);
+        af[f][0] = p[(2 * gq) * D_BM];
+        af[f][1] = p[(2 * gq + 1) * D_BM];
+      } else {
+        const v2d v = *(const v2d*)(cur + (wy * 64 + f * 16) * 64 + foff);
+        af[f][0] = v.x;
+        af[f][1] = v.y;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < C::FN; ++f) {
+      if (BKN) {
+        const double* p = (const double*)(cur + C::PA) + (wx * C::WN + f * 16 + l16);
+        bf[f][0] = p[(2 * gq) * BN];
+        bf[f][1] = p[(2 * gq + 1) * BN];
+      } else {
+        const v2d v = *(const v2d*)(cur + C::PA + (wx * C::WN + f * 16) * 64 + foff);
+        bf[f][0] = v.x;
+        bf[f][1] = v.y;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni)
+          acc[mi][ni] =
+              __builtin_amdgcn_mfma_f64_16x16x4f64(af[mi][s], bf[ni][s], acc[mi][ni], 0, 0, 0);
+  }
+  __syncthreads();
+}
+
+bool gemm_d_supported(const GemmArgs& g);
+// Returns BTA_OK after launching, a negative value when the shape is left to
+// the generic kernel, or an error code.
+int gemm_d(const GemmArgs& g, int batch, hipStream_t s);
+
+}  // namespace bta

@@ -26,6 +26,7 @@
 // worker count, and workers pair heavy with light tiles on each CU.
 #include "bta_common.h"
 #include "gemm.h"
+#include "gemm_d.h"
 #include "gemm_tile.h"
 #include "kernels.h"
 
@@ -38,7 +39,6 @@
 namespace bta {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 constexpr int H2_BM = 128;
 constexpr int H2_BK = 32;        // K per chunk (one MFMA K-step)
@@ -70,17 +70,6 @@ template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
-}
-
-// Tile of position p in round j of W workers: rounds alternate direction
-// (snake), and within a round the second half of the workers takes the tiles
-// in reverse, so workers p and p + W/2 (one CU when two workgroups share it)
-// hold a heavy and a light tile.
-__host__ __device__ inline int h2_tile_of(int j, int p, int W) {
-  const int h = W / 2;
-  int q = (W > 1 && p >= h) ? W - 1 - (p - h) : p;
-  if (j & 1) q = W - 1 - q;
-  return j * W + q;
 }
 
 template <int BN, bool ACC64>

@@ -17,11 +17,20 @@ int covar_grad(int kind, const double* X1s, int64_t n1, const double* X2s, int64
 int covar_ext(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2, int d,
               double s, const double* E, int ne, double* P, hipStream_t st);
 size_t potrf_ws_bytes(int64_t n_pad);
+// potrf options: POTRF_NO_SAVE keeps no copy of the input (a failure is
+// returned at once; the caller rebuilds the matrix with jitter);
+// POTRF_KEEP_UPPER leaves the strict upper triangle as it was.  With Linv, the
+// 64 x 64 diagonal blocks of L^-1 are written and the rest of its upper
+// triangle zeroed.
+constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2;
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
-          double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s);
+          double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
+          int opts = 0);
 size_t trtri_ws_bytes(int64_t n_pad);
+// TRTRI_UPPER_ZERO: X's strict upper triangle is already zero.
+constexpr int TRTRI_UPPER_ZERO = 1;
 int trtri_lower(const double* L, int64_t ldl, double* X, int64_t ldx, int64_t n_pad, void* ws,
-                size_t ws_bytes, hipStream_t s);
+                size_t ws_bytes, hipStream_t s, int opts = 0);
 template <typename TO>
 int cast_pad(const double* in, int64_t ldi, int64_t rows, int64_t cols, TO* out, int64_t ldo,
              int64_t rows_pad, int64_t cols_pad, hipStream_t s);

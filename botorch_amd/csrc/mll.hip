@@ -4,41 +4,66 @@
 //   log p(y) = -1/2 (|t|^2 + 2 sum log L_ii + n log 2 pi)
 //   d log p / d theta = 1/2 sum_ij W_ij dK_ij/dtheta,   W = a a^T - K^-1
 //
-// K^-1 = L^-T L^-1 is never stored: mll_trace_kernel computes each lower
-// tile of it on the MFMA GEMM mainloop (both operands read straight from L^-1,
-// the K-range starting at the tile's larger block index), then the epilogue
-// rebuilds k and dk/dtheta for the tile's point pairs from the scaled inputs
-// staged in LDS and folds W_ij dK_ij/dtheta into per-worker sums.  The
-// n^2 x (d+2) derivative tensor is never formed either.
+// K^-1 = X^T X with X = L^-1 is never stored: mll_trace_kernel forms each
+// lower 128 x 128 tile of it on the FP64 MFMA main loop (d_mainloop, both
+// operands read K-major straight from X), the K range of a tile,
+// [128 tm, n_pad), cut into chunks of KC so that every task costs about the
+// same.  Each task's epilogue rebuilds k and dk/dtheta for the tile's point
+// pairs from the scaled inputs staged in LDS and folds -P_ij dK_ij/dtheta
+// (plus a_i a_j dK_ij/dtheta for the tile's first chunk; the sum is linear in
+// K^-1) into per-worker sums, so the n^2 x (d + 2) derivative tensor is never
+// formed either.  a comes from two triangular matrix-vector products with X.
 // Reference loss: optim/closures/model_closures.py:125-145 (ExactMarginal-
 // LogLikelihood, gpytorch); derivative: Rasmussen & Williams (2006) eq. 5.9.
 #include "bta_common.h"
 #include "gemm.h"
+#include "gemm_d.h"
 #include "gemm_tile.h"
 #include "kernels.h"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace bta {
 
+constexpr int TR_B = 128;   // trace tile
+constexpr int TR_KC = 512;  // K chunk per task
+
 struct TraceArgs {
-  const double* linv;
+  const double* X;  // L^-1 (lower), n_pad x n_pad, row stride ld
   int64_t ld;
-  const double* xs;
-  const double* alpha;
-  double* partial;  // gridDim.x x (MD + 2)
-  int n_pad, n, d, kind;
+  const double* xs;     // n x d scaled inputs
+  const double* alpha;  // n_pad
+  double* partial;      // gridDim.x x (MD + 2)
+  int n_pad, n, d, MT, CB;
   double s;
 };
 
-template <int B, int BK, int MD>
-__global__ __launch_bounds__(256) void mll_trace_kernel(TraceArgs a) {
-  typedef Cfg<double, B, B, BK, true, true> C;
-  static_assert(2 * C::STAGE >= 2 * B * MD, "xs staging must fit in the mainloop LDS");
-  __shared__ __attribute__((aligned(16))) double lds[2 * C::STAGE];
-  const int MT = a.n_pad / B;
-  const int total = MT * (MT + 1) / 2;
+// Tasks of chunk c cover the tiles tm < MT - c CB (those whose K range
+// reaches chunk c), lower tiles in decode_tile's C_LOWER order.
+__device__ __forceinline__ bool trace_task(int t, int MT, int CB, int& tm, int& tn, int& c) {
+  for (c = 0;; ++c) {
+    const int T = MT - c * CB;
+    if (T <= 0) return false;
+    const int cnt = T * (T + 1) / 2;
+    if (t < cnt) break;
+    t -= cnt;
+  }
+  const int T = MT - c * CB;
+  decode_tile(t, BTA_GEMM_C_LOWER, T, T, tm, tn);
+  return true;
+}
+
+template <int MD, int KIND>
+__global__ __launch_bounds__(256, 1) void mll_trace_kernel(TraceArgs a) {
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[DCfg<128>::STAGES * DCfg<128>::STAGE];
+  static_assert(2 * TR_B * MD * 8 <= DCfg<128>::STAGES * DCfg<128>::STAGE, "x staging");
+  int total = 0;
+  for (int c = 0; a.MT - c * a.CB > 0; ++c) {
+    const int T = a.MT - c * a.CB;
+    total += T * (T + 1) / 2;
+  }
   const int W = gridDim.x, w = blockIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
@@ -47,65 +72,81 @@ __global__ __launch_bounds__(256) void mll_trace_kernel(TraceArgs a) {
 #pragma unroll
   for (int j = 0; j < MD; ++j) acc_ls[j] = 0.0;
 
-  for (int jj = 0;; ++jj) {
-    const int t = (jj & 1) ? (jj + 1) * W - 1 - w : jj * W + w;
-    if (t >= total) break;
-    int tm, tn;
-    decode_tile(t, BTA_GEMM_C_LOWER, MT, MT, tm, tn);
-    const int kb = tm * B;  // tm >= tn: both L^-1 columns vanish above row tm*B
-    v4d acc[C::MI][C::NI];
-    gemm_tile<double, B, B, BK, true, true>(a.linv, a.ld, a.linv, a.ld, tm * B, tn * B, kb,
-                                            a.n_pad, lds, acc);
-    // stage the tile's scaled rows and columns (mainloop LDS is free now)
-    double* xr = lds;
-    double* xc = lds + B * MD;
-    for (int e = threadIdx.x; e < B * d; e += 256) {
-      const int r = e / d, j = e % d;
-      const int gi = tm * B + r, gj = tn * B + r;
-      xr[r * MD + j] = gi < a.n ? a.xs[(int64_t)gi * d + j] : 0.0;
-      xc[r * MD + j] = gj < a.n ? a.xs[(int64_t)gj * d + j] : 0.0;
+  for (int jj = 0; jj * W < total; ++jj) {
+    const int t = h2_tile_of(jj, w, W);
+    int tm, tn, c;
+    if (t >= total || !trace_task(t, a.MT, a.CB, tm, tn, c)) continue;
+    const int kb = tm * TR_B + c * TR_KC;
+    const int ke = min(a.n_pad, kb + TR_KC);
+    v4d acc[4][4];
+    d_mainloop<128, true, true>(a.X, a.ld, a.X, a.ld, tm * TR_B, tn * TR_B, a.n_pad, a.n_pad, kb,
+                                ke, lds, acc);
+    // the ring is free: stage the tile's scaled rows and columns
+    double* xr = (double*)lds;
+    double* xc = xr + TR_B * MD;
+    for (int e = threadIdx.x; e < TR_B * MD; e += 256) {
+      const int r = e / MD, j = e % MD;
+      const int gi = tm * TR_B + r, gj = tn * TR_B + r;
+      xr[e] = (j < d && gi < a.n) ? a.xs[(int64_t)gi * d + j] : 0.0;
+      xc[e] = (j < d && gj < a.n) ? a.xs[(int64_t)gj * d + j] : 0.0;
     }
     __syncthreads();
+    const bool first = c == 0;
 #pragma unroll
-    for (int ni = 0; ni < C::NI; ++ni) {
-      const int cl = wx * (B / 2) + ni * 16 + l16;
-      const int gj = tn * B + cl;
-      if (gj >= a.n) continue;
-      const double aj = a.alpha[gj];
+    for (int ni = 0; ni < 4; ++ni) {
+      const int cl = wx * 64 + ni * 16 + l16;
+      const int gj = tn * TR_B + cl;
+      const bool jok = gj < a.n;
+      const double aj = jok ? a.alpha[gj] : 0.0;
       double xj[MD];
 #pragma unroll
-      for (int j = 0; j < MD; ++j) xj[j] = j < d ? xc[cl * MD + j] : 0.0;
+      for (int j = 0; j < MD; ++j) xj[j] = xc[cl * MD + j];
 #pragma unroll
-      for (int mi = 0; mi < C::MI; ++mi)
-#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll 1
         for (int q = 0; q < 4; ++q) {
-          const int rl = wy * (B / 2) + mi * 16 + Mfma<double>::row(gq, q);
-          const int gi = tm * B + rl;
-          if (gi >= a.n || gi < gj) continue;
-          const double wij = a.alpha[gi] * aj - acc[mi][ni][q];
-          const double f = gi == gj ? 1.0 : 2.0;
-          double r2 = 0.0;
-          double dl[MD];
+          // (a runtime q keeps one element in flight: the epilogue then fits
+          // the register file next to the accumulators)
+          const v4d& av = acc[mi][ni];
+          const double pij = q == 0 ? av[0] : q == 1 ? av[1] : q == 2 ? av[2] : av[3];
+          const int rl = wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
+          const int gi = tm * TR_B + rl;
+          const bool ok = jok && gi < a.n && gi >= gj;
+          const double wij = (first ? a.alpha[min(gi, a.n_pad - 1)] * aj : 0.0) - pij;
+          const double f = ok ? (gi == gj ? 1.0 : 2.0) : 0.0;
+          double r2 = 0.0, xi[MD];
 #pragma unroll
           for (int j = 0; j < MD; ++j) {
-            const double df = j < d ? xr[rl * MD + j] - xj[j] : 0.0;
-            dl[j] = df * df;
-            r2 += dl[j];
+            xi[j] = xr[rl * MD + j];
+            const double df = xi[j] - xj[j];
+            r2 = fma(df, df, r2);
           }
-          double kv;
-          const double h = kern_grad_factor(a.kind, r2, a.s, &kv);
+          double kv, h;
+          if (KIND == BTA_KERNEL_MATERN52) {
+            const double r = sqrt_pos(fmax(r2, 1e-30));
+            const double sr = 2.2360679774997896964 * r;
+            const double e = a.s * exp_nonpos(-sr);
+            kv = (1.0 + sr + (5.0 / 3.0) * r2) * e;
+            h = (5.0 / 3.0) * (1.0 + sr) * e;
+          } else {
+            kv = a.s * exp_nonpos(-0.5 * r2);
+            h = kv;
+          }
           const double fw = f * wij;
           acc_os = fma(fw, kv, acc_os);
-          if (gi == gj) acc_noise += wij;
+          acc_noise += (ok && gi == gj) ? wij : 0.0;
           const double fwh = fw * h;
 #pragma unroll
-          for (int j = 0; j < MD; ++j) acc_ls[j] = fma(fwh, dl[j], acc_ls[j]);
+          for (int j = 0; j < MD; ++j) {
+            const double df = xi[j] - xj[j];
+            acc_ls[j] = fma(fwh, df * df, acc_ls[j]);
+          }
         }
     }
     __syncthreads();
   }
   // per-worker reduction
-  double* red = lds;
+  double* red = (double*)lds;
   double v = wave_sum(acc_os);
   if (lane == 0) red[wv * (MD + 2)] = v;
   v = wave_sum(acc_noise);
@@ -121,51 +162,105 @@ __global__ __launch_bounds__(256) void mll_trace_kernel(TraceArgs a) {
         red[e] + red[(MD + 2) + e] + red[2 * (MD + 2) + e] + red[3 * (MD + 2) + e];
 }
 
-// out[0] = sum t_i^2, out[1] = sum_{i<n} log L_ii, out[2] = sum_{i<n} alpha_i,
-// out[3..] = sum over workers of the trace partials (MD + 2 values).
+// t = X r for the lower-triangular X (one wave per row, double2 loads)
+__global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restrict__ X, int64_t ld,
+                                                         const double* __restrict__ r, int64_t n,
+                                                         double* __restrict__ t) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double* row = X + i * ld;
+  double acc = 0.0;
+  for (int64_t k = 2 * lane; k <= i; k += 128) {
+    const v2d x = *(const v2d*)(row + k);
+    acc = fma(x.x, r[k], acc);
+    if (k + 1 <= i) acc = fma(x.y, r[k + 1], acc);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) t[i] = acc;
+}
+
+// part[rb][j] = sum over the 64 rows i of block rb (i >= j) of X_ij t_i
+__global__ __launch_bounds__(256) void trmv_lower_t_part_kernel(const double* __restrict__ X,
+                                                                int64_t ld,
+                                                                const double* __restrict__ t,
+                                                                int64_t n_pad,
+                                                                double* __restrict__ part) {
+  __shared__ double ts[64];
+  const int rb = blockIdx.y;
+  const int64_t i0 = (int64_t)rb * 64;
+  if (threadIdx.x < 64) ts[threadIdx.x] = t[i0 + threadIdx.x];
+  __syncthreads();
+  const int64_t jend = i0 + 64;
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < jend; j += (int64_t)gridDim.x * 256) {
+    double acc = 0.0;
+#pragma unroll 8
+    for (int r = 0; r < 64; ++r) {
+      const int64_t i = i0 + r;
+      acc = fma(i >= j ? X[i * ld + j] : 0.0, ts[r], acc);
+    }
+    part[(int64_t)rb * n_pad + j] = acc;
+  }
+}
+
+// a_j = sum_{rb >= j / 64} part[rb][j]
+__global__ void trmv_lower_t_sum_kernel(const double* __restrict__ part, int64_t n_pad,
+                                        double* __restrict__ a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_pad) return;
+  double s = 0.0;
+  for (int64_t rb = j / 64; rb < n_pad / 64; ++rb) s += part[rb * n_pad + j];
+  a[j] = s;
+}
+
+// out[0] = sum t_i^2, out[1] = sum_{i<n} log L_ii, out[2] = sum_{i<n} a_i,
+// out[3 + e] = sum over the W workers of partial[w][e], e < np2.
 __global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const double* L,
                                                          int64_t ld, const double* alpha,
                                                          int64_t n, const double* partial,
                                                          int W, int np2, double* out) {
-  __shared__ double red[4 * 3];
+  __shared__ double red[8 * 40];
+  __shared__ double red3[4 * 3];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double q = 0.0, lg = 0.0, sa = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 256) {
+  for (int64_t i = tid; i < n; i += 256) {
     q += t[i] * t[i];
     lg += log(L[i * ld + i]);
     sa += alpha[i];
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   q = wave_sum(q);
   lg = wave_sum(lg);
   sa = wave_sum(sa);
   if (lane == 0) {
-    red[wv * 3] = q;
-    red[wv * 3 + 1] = lg;
-    red[wv * 3 + 2] = sa;
+    red3[wv * 3] = q;
+    red3[wv * 3 + 1] = lg;
+    red3[wv * 3 + 2] = sa;
+  }
+  // worker sums: group gi of 32 threads takes workers w = gi (mod 8)
+  const int gi = tid >> 5, e0 = tid & 31;
+  for (int e = e0; e < np2; e += 32) {
+    double acc = 0.0;
+    for (int w = gi; w < W; w += 8) acc += partial[(int64_t)w * np2 + e];
+    red[gi * 40 + e] = acc;
   }
   __syncthreads();
-  if (threadIdx.x < 3)
-    out[threadIdx.x] = red[threadIdx.x] + red[3 + threadIdx.x] + red[6 + threadIdx.x] +
-                       red[9 + threadIdx.x];
-  for (int e = threadIdx.x; e < np2; e += 256) {
-    double s = 0.0;
-    for (int w = 0; w < W; ++w) s += partial[(int64_t)w * np2 + e];
-    out[3 + e] = s;
+  if (tid < 3) out[tid] = red3[tid] + red3[3 + tid] + red3[6 + tid] + red3[9 + tid];
+  for (int e = tid; e < np2; e += 256) {
+    double acc = 0.0;
+    for (int g = 0; g < 8; ++g) acc += red[g * 40 + e];
+    out[3 + e] = acc;
   }
 }
 
-__global__ void residual_pad_kernel(const double* y, int64_t n, double c, double* R,
-                                    int64_t n_pad) {
-  // R is 64 x n_pad; row 0 = y - c (zero padded), other rows zero
+__global__ void mll_residual_kernel(const double* y, int64_t n, double c, double* r, int64_t n_pad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 64 * n_pad) return;
-  R[i] = i < n ? y[i] - c : 0.0;
+  if (i < n_pad) r[i] = i < n ? y[i] - c : 0.0;
 }
 
 struct MllWs {
   int64_t n_pad;
   int W, MD;
-  size_t ls, center, xs, L, Linv, fact, R, T, Aw, partial, out, total;
+  size_t ls, center, xs, L, Linv, fact, r, t, alpha, part, partial, out, total;
   MllWs(int64_t n, int d) {
     n_pad = round_up(n, 128);
     MD = d <= 16 ? 16 : 32;
@@ -177,20 +272,29 @@ struct MllWs {
     L = p.take((size_t)n_pad * n_pad * 8);
     Linv = p.take((size_t)n_pad * n_pad * 8);
     fact = p.take(std::max(potrf_ws_bytes(n_pad), trtri_ws_bytes(n_pad)));
-    R = p.take((size_t)64 * n_pad * 8);
-    T = p.take((size_t)64 * n_pad * 8);
-    Aw = p.take((size_t)64 * n_pad * 8);
+    r = p.take((size_t)n_pad * 8);
+    t = p.take((size_t)n_pad * 8);
+    alpha = p.take((size_t)n_pad * 8);
+    part = p.take((size_t)(n_pad / 64) * n_pad * 8);
     partial = p.take((size_t)W * (MD + 2) * 8);
     out = p.take((size_t)(MD + 8) * 8);
     total = p.off;
   }
 };
 
+template <int MD>
+static void launch_trace(int kind, const TraceArgs& ta, int W, hipStream_t st) {
+  if (kind == BTA_KERNEL_MATERN52)
+    hipLaunchKernelGGL((mll_trace_kernel<MD, BTA_KERNEL_MATERN52>), dim3(W), dim3(256), 0, st, ta);
+  else
+    hipLaunchKernelGGL((mll_trace_kernel<MD, BTA_KERNEL_RBF>), dim3(W), dim3(256), 0, st, ta);
+}
+
 int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
                    const double* ls_host, double s, double noise, double c, int max_tries,
                    double* out_host, void* ws, size_t ws_bytes, hipStream_t st) {
   if (!X || !y || !ls_host || !out_host || n < 2 || d < 1 || d > 32 || !(s > 0) ||
-      !(noise >= 0)) {
+      !(noise >= 0) || (kind != BTA_KERNEL_MATERN52 && kind != BTA_KERNEL_RBF)) {
     set_error("mll_value_grad: invalid arguments");
     return BTA_ERR_INVALID;
   }
@@ -207,9 +311,10 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   double* Linv = (double*)(base + w.Linv);
   void* fact = base + w.fact;
   const size_t fact_bytes = std::max(potrf_ws_bytes(w.n_pad), trtri_ws_bytes(w.n_pad));
-  double* R = (double*)(base + w.R);
-  double* T = (double*)(base + w.T);
-  double* Aw = (double*)(base + w.Aw);
+  double* r = (double*)(base + w.r);
+  double* t = (double*)(base + w.t);
+  double* alpha = (double*)(base + w.alpha);
+  double* part = (double*)(base + w.part);
   double* partial = (double*)(base + w.partial);
   double* out = (double*)(base + w.out);
   const int64_t np = w.n_pad;
@@ -217,45 +322,66 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   BTA_HIP(hipMemcpyAsync(ls, ls_host, d * sizeof(double), hipMemcpyHostToDevice, st));
   int rc = scale_inputs(X, n, d, nullptr, ls, center, xs, np, st);
   if (rc) return rc;
-  rc = covar(kind, xs, n, xs, n, d, s, noise, false, L, np, np, np, st, /*pad_identity=*/1);
+  // psd_safe_cholesky's schedule: no jitter, then jitter0 * 10^(i-1); K is
+  // rebuilt for each attempt instead of being saved
+  const double jitter0 = 1e-8;
+  for (int attempt = 0;; ++attempt) {
+    const double jit = attempt == 0 ? 0.0 : jitter0 * std::pow(10.0, (double)(attempt - 1));
+    rc = covar(kind, xs, n, xs, n, d, s, noise + jit, false, L, np, np, np, st,
+               /*pad_identity=*/1);
+    if (rc) return rc;
+    double used = 0.0;
+    rc = potrf(L, np, np, Linv, np, 0, jitter0, &used, fact, fact_bytes, st,
+               POTRF_NO_SAVE | POTRF_KEEP_UPPER);
+    if (rc != BTA_ERR_NOT_PSD) break;
+    if (attempt >= max_tries) {
+      set_error("mll_value_grad: matrix not positive definite after adding jitter up to %.1e",
+                jit);
+      return BTA_ERR_NOT_PSD;
+    }
+  }
   if (rc) return rc;
-  double jit = 0.0;
-  rc = potrf(L, np, np, Linv, np, max_tries, 1e-8, &jit, fact, fact_bytes, st);
+  rc = trtri_lower(L, np, Linv, np, np, fact, fact_bytes, st, TRTRI_UPPER_ZERO);
   if (rc) return rc;
-  rc = trtri_lower(L, np, Linv, np, np, fact, fact_bytes, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(residual_pad_kernel, dim3((unsigned)((64 * np + 255) / 256)), dim3(256), 0,
-                     st, y, n, c, R, np);
-  BTA_LAUNCH_CHECK("residual_pad_kernel");
-  rc = gemm(BTA_GEMM_B_LOWER, 64, np, np, 1.0, R, np, Linv, np, 0.0, nullptr, 0, T, np, st);
-  if (rc) return rc;
-  rc = gemm(BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, 64, np, np, 1.0, T, np, Linv, np, 0.0, nullptr, 0,
-            Aw, np, st);
-  if (rc) return rc;
+  hipLaunchKernelGGL(mll_residual_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, y, n,
+                     c, r, np);
+  BTA_LAUNCH_CHECK("mll_residual_kernel");
+  hipLaunchKernelGGL(trmv_lower_kernel, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, Linv,
+                     np, r, np, t);
+  BTA_LAUNCH_CHECK("trmv_lower_kernel");
+  {
+    const unsigned gx = (unsigned)std::min<int64_t>((np + 255) / 256, 16);
+    hipLaunchKernelGGL(trmv_lower_t_part_kernel, dim3(gx, (unsigned)(np / 64)), dim3(256), 0, st,
+                       Linv, np, t, np, part);
+    BTA_LAUNCH_CHECK("trmv_lower_t_part_kernel");
+    hipLaunchKernelGGL(trmv_lower_t_sum_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0,
+                       st, part, np, alpha);
+    BTA_LAUNCH_CHECK("trmv_lower_t_sum_kernel");
+  }
   TraceArgs ta;
-  ta.linv = Linv;
+  ta.X = Linv;
   ta.ld = np;
   ta.xs = xs;
-  ta.alpha = Aw;
+  ta.alpha = alpha;
   ta.partial = partial;
   ta.n_pad = (int)np;
   ta.n = (int)n;
   ta.d = d;
-  ta.kind = kind;
+  ta.MT = (int)(np / TR_B);
+  ta.CB = TR_KC / TR_B;
   ta.s = s;
-  const int MT128 = (int)(np / 128), MT64 = (int)(np / 64);
-  int W;
-  if (d <= 16) {
-    const int total = MT128 * (MT128 + 1) / 2;
-    W = std::min(w.W, total);
-    hipLaunchKernelGGL((mll_trace_kernel<128, 8, 16>), dim3(W), dim3(256), 0, st, ta);
-  } else {
-    const int total = MT64 * (MT64 + 1) / 2;
-    W = std::min(w.W, total);
-    hipLaunchKernelGGL((mll_trace_kernel<64, 16, 32>), dim3(W), dim3(256), 0, st, ta);
+  int tasks = 0;
+  for (int cc = 0; ta.MT - cc * ta.CB > 0; ++cc) {
+    const int T = ta.MT - cc * ta.CB;
+    tasks += T * (T + 1) / 2;
   }
+  const int W = std::min(w.W, tasks);
+  if (d <= 16)
+    launch_trace<16>(kind, ta, W, st);
+  else
+    launch_trace<32>(kind, ta, W, st);
   BTA_LAUNCH_CHECK("mll_trace_kernel");
-  hipLaunchKernelGGL(mll_finish_kernel, dim3(1), dim3(256), 0, st, T, L, np, Aw, n, partial, W,
+  hipLaunchKernelGGL(mll_finish_kernel, dim3(1), dim3(256), 0, st, t, L, np, alpha, n, partial, W,
                      w.MD + 2, out);
   BTA_LAUNCH_CHECK("mll_finish_kernel");
   std::vector<double> h(3 + w.MD + 2);

@@ -140,6 +140,28 @@ def test_potrf_and_trtri(n):
     assert torch.triu(Linv, 1).abs().max().item() == 0.0
 
 
+def test_potrf_large_repeated():
+    """n = 4096: the factorisation's step launches then hold more workgroups than
+    CUs (strip and trailing-update workgroups share CUs); three runs must all
+    agree with torch's Cholesky."""
+    _say("potrf n=4096 x3")
+    L = _lib.lib()
+    n = 4096
+    g = torch.Generator(device=DEV).manual_seed(7)
+    B = torch.randn(n, n, dtype=torch.float64, device=DEV, generator=g)
+    A0 = B @ B.T / n + torch.eye(n, dtype=torch.float64, device=DEV)
+    Lref = torch.linalg.cholesky(A0)
+    ws = _lib.workspace(L.bta_potrf_workspace_size(n), DEV)
+    Linv = torch.zeros_like(A0)
+    for _ in range(3):
+        A = A0.clone()
+        jit = ctypes.c_double(-1)
+        _lib.check(L.bta_potrf(_lib.ptr(A), n, n, _lib.ptr(Linv), n, 6, 1e-8, ctypes.byref(jit),
+                               _lib.ptr(ws), ws.numel(), _lib.stream()))
+        err = ((A - Lref).abs().max() / Lref.abs().max()).item()
+        assert err < 1e-12, err
+
+
 def test_potrf_jitter_schedule():
     _say("potrf jitter on a singular matrix")
     L = _lib.lib()
