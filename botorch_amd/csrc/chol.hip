@@ -453,20 +453,31 @@ __device__ __forceinline__ void potrf_bulk(const StepArgs& a, unsigned char* sme
     v4d acc[4][4];
     d_mainloop<128, false, false>(P, lda, P, lda, tm * D_BM, tn * D_BM, rem, rem, 0, LEAF, smem,
                                   acc);
+    // per 16-row block, all loads before the stores (a store could alias a
+    // later load, which would otherwise serialise every element on memory
+    // latency)
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < 4; ++mi) {
+      double cv[4][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
-        if (row >= rem) continue;
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
           const int col = tn * D_BM + wx * 64 + ni * 16 + l16;
-          if (col > row) continue;
-          double* c = Cm + (int64_t)row * lda + col;
-          *c -= acc[mi][ni][q];
+          cv[ni][q] = (row < rem && col <= row) ? Cm[(int64_t)row * lda + col] : 0.0;
         }
       }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+          const int col = tn * D_BM + wx * 64 + ni * 16 + l16;
+          if (row < rem && col <= row) Cm[(int64_t)row * lda + col] = cv[ni][q] - acc[mi][ni][q];
+        }
+      }
+    }
   }
 }
 

@@ -36,6 +36,32 @@ __device__ __forceinline__ void d_wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
+// LDS fragment reads as inline asm.  The compiler then sees no LDS read after
+// the direct loads, so it adds no s_waitcnt vmcnt(0) in front of every chunk's
+// fragment reads (which would drain the whole ring each iteration); the
+// explicit vmcnt wait and the barrier in the main loop order the data, and
+// lds_frag_wait() waits for the reads themselves.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void*)p;
+}
+template <int OFF>
+__device__ __forceinline__ v2d lds_read_b128(uint32_t a) {
+  v2d v;
+  __asm__ __volatile__("ds_read_b128 %0, %1 offset:%2" : "=v"(v) : "v"(a), "i"(OFF));
+  return v;
+}
+template <int OFF>
+__device__ __forceinline__ double lds_read_b64(uint32_t a) {
+  double v;
+  __asm__ __volatile__("ds_read_b64 %0, %1 offset:%2" : "=v"(v) : "v"(a), "i"(OFF));
+  return v;
+}
+__device__ __forceinline__ void lds_frag_wait() {
+  __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+}
+// ties v to the wait above: nothing using v is scheduled before it
+__device__ __forceinline__ void lds_tie(double& v) { __asm__ __volatile__("" : "+v"(v)); }
+
 // Tile of position p in round j of W workers: rounds alternate direction, and
 // within a round the second half of the workers takes the tiles in reverse, so
 // workers p and p + W/2 (one CU when two workgroups share it) pair a heavy and
@@ -45,6 +71,31 @@ __host__ __device__ inline int h2_tile_of(int j, int p, int W) {
   int q = (W > 1 && p >= h) ? W - 1 - (p - h) : p;
   if (j & 1) q = W - 1 - q;
   return j * W + q;
+}
+
+// F fragments of 16 rows each from a K-contiguous stage (64-byte rows): one
+// b128 per fragment at a0 + f * 1024.
+template <int F>
+__device__ __forceinline__ void frag_nk(uint32_t a0, double (&x)[F][2]) {
+  v2d v[F];
+  if (F > 0) v[0] = lds_read_b128<0>(a0);
+  if (F > 1) v[1] = lds_read_b128<1024>(a0);
+  if (F > 2) v[2] = lds_read_b128<2048>(a0);
+  if (F > 3) v[3] = lds_read_b128<3072>(a0);
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    x[f][0] = v[f].x;
+    x[f][1] = v[f].y;
+  }
+}
+// F fragments from a K-major stage of W doubles per k row: k = 2g and 2g + 1
+// (a0 points at row 2g), fragment f at +16 f doubles.
+template <int F, int W>
+__device__ __forceinline__ void frag_mn(uint32_t a0, double (&x)[F][2]) {
+  if (F > 0) { x[0][0] = lds_read_b64<0>(a0); x[0][1] = lds_read_b64<W * 8>(a0); }
+  if (F > 1) { x[1][0] = lds_read_b64<128>(a0); x[1][1] = lds_read_b64<W * 8 + 128>(a0); }
+  if (F > 2) { x[2][0] = lds_read_b64<256>(a0); x[2][1] = lds_read_b64<W * 8 + 256>(a0); }
+  if (F > 3) { x[3][0] = lds_read_b64<384>(a0); x[3][1] = lds_read_b64<W * 8 + 384>(a0); }
 }
 
 // acc = op(A)[row0 : row0+128, kb:ke] op(B)[kb:ke, col0 : col0+BN] for the
@@ -124,35 +175,41 @@ __device__ __forceinline__ void d_mainloop(const double* __restrict__ A, int64_t
     // every wave's loads of chunk it landed and every wave is done with chunk
     // it - 1 (whose stage the next issue overwrites).  A bare s_barrier: the
     // fence of __syncthreads would wait for the loads still in flight.
+    // sched_barrier(0): the compiler moves nothing across, in particular no
+    // LDS read of this chunk above the wait and the barrier (another wave's
+    // direct loads fill part of the stage, and only the barrier orders them)
+    __builtin_amdgcn_sched_barrier(0);
     __asm__ __volatile__("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     __asm__ __volatile__("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
     if (it + S - 1 < nk) issue(it + S - 1);
     const unsigned char* cur = lds + (it % S) * C::STAGE;
     double af[C::FM][2], bf[C::FN][2];
+    if (AKM) {
+      const uint32_t a0 = lds_addr((const double*)cur + (wy * 64 + l16) + 2 * gq * D_BM);
+      frag_mn<C::FM, D_BM>(a0, af);
+    } else {
+      const uint32_t a0 = lds_addr(cur + wy * 64 * 64 + foff);
+      frag_nk<C::FM>(a0, af);
+    }
+    if (BKN) {
+      const uint32_t b0 = lds_addr((const double*)(cur + C::PA) + (wx * C::WN + l16) + 2 * gq * BN);
+      frag_mn<C::FN, BN>(b0, bf);
+    } else {
+      const uint32_t b0 = lds_addr(cur + C::PA + wx * C::WN * 64 + foff);
+      frag_nk<C::FN>(b0, bf);
+    }
+    lds_frag_wait();
 #pragma unroll
     for (int f = 0; f < C::FM; ++f) {
-      if (AKM) {
-        const double* p = (const double*)cur + (wy * 64 + f * 16 + l16);
-        af[f][0] = p[(2 * gq) * D_BM];
-        af[f][1] = p[(2 * gq + 1) * D_BM];
-      } else {
-        const v2d v = *(const v2d*)(cur + (wy * 64 + f * 16) * 64 + foff);
-        af[f][0] = v.x;
-        af[f][1] = v.y;
-      }
+      lds_tie(af[f][0]);
+      lds_tie(af[f][1]);
     }
 #pragma unroll
     for (int f = 0; f < C::FN; ++f) {
-      if (BKN) {
-        const double* p = (const double*)(cur + C::PA) + (wx * C::WN + f * 16 + l16);
-        bf[f][0] = p[(2 * gq) * BN];
-        bf[f][1] = p[(2 * gq + 1) * BN];
-      } else {
-        const v2d v = *(const v2d*)(cur + C::PA + (wx * C::WN + f * 16) * 64 + foff);
-        bf[f][0] = v.x;
-        bf[f][1] = v.y;
-      }
+      lds_tie(bf[f][0]);
+      lds_tie(bf[f][1]);
     }
 #pragma unroll
     for (int s = 0; s < 2; ++s)

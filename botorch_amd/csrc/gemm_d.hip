@@ -61,8 +61,22 @@ __global__ __launch_bounds__(256, 2) void gemm_d_kernel(GemmArgs g) {
     v4d acc[C::FM][C::FN];
     d_mainloop<BN, AKM, BKN>(A, g.lda, B, g.ldb, tm * D_BM, tn * BN, g.M, g.N, kb, ke, lds,
                              acc);
+    // C0 may alias C: per 16-row block, its C0 loads go before its stores, so
+    // they overlap instead of each waiting behind the previous store
 #pragma unroll
-    for (int mi = 0; mi < C::FM; ++mi)
+    for (int mi = 0; mi < C::FM; ++mi) {
+      double c0v[C::FN][4];
+      if (C0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
+#pragma unroll
+          for (int ni = 0; ni < C::FN; ++ni) {
+            const int64_t col = tn * BN + wx * C::WN + ni * 16 + l16;
+            c0v[ni][q] = (row < g.M && col < g.N) ? C0[row * g.ldc0 + col] : 0.0;
+          }
+        }
+      }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int64_t row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
@@ -72,10 +86,11 @@ __global__ __launch_bounds__(256, 2) void gemm_d_kernel(GemmArgs g) {
           const int64_t col = tn * BN + wx * C::WN + ni * 16 + l16;
           if (col >= g.N) continue;
           double v = g.alpha * acc[mi][ni][q];
-          if (C0) v += g.beta * C0[row * g.ldc0 + col];
+          if (C0) v += g.beta * c0v[ni][q];
           Cp[row * g.ldc + col] = v;
         }
       }
+    }
   }
 }
 

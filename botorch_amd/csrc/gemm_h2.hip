@@ -72,6 +72,23 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
+template <int OFF>
+__device__ __forceinline__ half8 h2_read(uint32_t a) {
+  half8 v;
+  __asm__ __volatile__("ds_read_b128 %0, %1 offset:%2" : "=v"(v) : "v"(a), "i"(OFF));
+  return v;
+}
+__device__ __forceinline__ void h2_tie(half8& v) { __asm__ __volatile__("" : "+v"(v)); }
+
+// F fragments (16 rows, 1 KiB apart) of a plane pair: hi at a, lo at a + P
+template <int F, int P>
+__device__ __forceinline__ void h2_frag(uint32_t a, half8 (&hi)[F], half8 (&lo)[F]) {
+  if (F > 0) { hi[0] = h2_read<0>(a); lo[0] = h2_read<P>(a); }
+  if (F > 1) { hi[1] = h2_read<1024>(a); lo[1] = h2_read<P + 1024>(a); }
+  if (F > 2) { hi[2] = h2_read<2048>(a); lo[2] = h2_read<P + 2048>(a); }
+  if (F > 3) { hi[3] = h2_read<3072>(a); lo[3] = h2_read<P + 3072>(a); }
+}
+
 template <int BN, bool ACC64>
 __global__ __launch_bounds__(256, (H2Cfg<BN, ACC64>::WG_PER_CU)) void gemm_h2_kernel(H2Args g) {
   using C = H2Cfg<BN, ACC64>;
@@ -153,9 +170,14 @@ __global__ __launch_bounds__(256, (H2Cfg<BN, ACC64>::WG_PER_CU)) void gemm_h2_ke
       // it - 1.  A bare s_barrier: the fence of __syncthreads would also wait for
       // the loads still in flight; the empty asm statements keep the compiler
       // from moving LDS accesses across it.
+      // sched_barrier(0): the compiler moves nothing across, in particular no
+      // LDS read of this chunk above the wait and the barrier (another wave's
+      // direct loads fill part of the stage, and only the barrier orders them)
+      __builtin_amdgcn_sched_barrier(0);
       __asm__ __volatile__("" ::: "memory");
       __builtin_amdgcn_s_barrier();
       __asm__ __volatile__("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
       if (it + S - 1 < nk) issue(it + S - 1);
       if (ACC64 && it % seg == 0 && it > 0) {
 #pragma unroll
@@ -167,18 +189,23 @@ __global__ __launch_bounds__(256, (H2Cfg<BN, ACC64>::WG_PER_CU)) void gemm_h2_ke
           }
       }
       const unsigned char* cur = lds + (it % S) * C::STAGE;
+      // fragment reads as inline asm (see lds_read_b128): no compiler-added
+      // vmcnt(0) for the direct loads in flight; lgkmcnt waited below
       half8 ah[C::FM], al[C::FM], bh[C::FN], bl[C::FN];
+      const uint32_t aa = lds_addr(cur + wy * 64 * H2_ROWB + foff);
+      const uint32_t ba = lds_addr(cur + 2 * C::PA + wx * C::WN * H2_ROWB + foff);
+      h2_frag<C::FM, C::PA>(aa, ah, al);
+      h2_frag<C::FN, C::PB>(ba, bh, bl);
+      lds_frag_wait();
 #pragma unroll
       for (int f = 0; f < C::FM; ++f) {
-        const int o = (wy * 64 + f * 16) * H2_ROWB + foff;
-        ah[f] = *(const half8*)(cur + o);
-        al[f] = *(const half8*)(cur + C::PA + o);
+        h2_tie(ah[f]);
+        h2_tie(al[f]);
       }
 #pragma unroll
       for (int f = 0; f < C::FN; ++f) {
-        const int o = (wx * C::WN + f * 16) * H2_ROWB + foff;
-        bh[f] = *(const half8*)(cur + 2 * C::PA + o);
-        bl[f] = *(const half8*)(cur + 2 * C::PA + C::PB + o);
+        h2_tie(bh[f]);
+        h2_tie(bl[f]);
       }
       // acc += A_hi B_hi, accx += A_lo B_hi + A_hi B_lo; three passes over the
       // fragments, so no MFMA waits on the one before it
