@@ -78,11 +78,8 @@ __device__ __forceinline__ double quad_perm_d(double v) {
 constexpr int QP_XOR1 = 0xB1;  // quad_perm [1, 0, 3, 2]
 constexpr int QP_XOR2 = 0x4E;  // quad_perm [2, 3, 0, 1]
 
-// Waits for this wave's outstanding LDS operations.  Lanes of one wave that
-// exchange data through LDS without a barrier need it between the write and
-// the reads (and before overwriting what other lanes still read): with a
-// second workgroup on the CU, a wave's LDS accesses are not reliably
-// performed in issue order.
+// Waits for this wave's outstanding LDS operations (a cheap guard where lanes
+// of one wave exchange data through LDS outside the hot loops).
 __device__ __forceinline__ void lds_wait() { __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 // One 16 x 16 x 16 FP64 MFMA product on LDS operands X (pitch PX), Y (pitch
@@ -181,11 +178,11 @@ __device__ __forceinline__ void chol64_blocked(double* S, double* rdiag, double*
         // rows above the panel and entries right of the diagonal take garbage;
         // only the lower part is kept
         if (c + 1 < 16) {
-          lds_wait();  // the previous column's reads of colv are done
+          // a wave's LDS accesses are performed in issue order: the reads see
+          // this column's colv and the previous column's reads are done
           colv[lane] = l;
           a[c + 1] = fma(-l, readlane_d(l, cc + 1), a[c + 1]);
           if (c + 2 < 16) {
-            lds_wait();
             double cv[16];
 #pragma unroll
             for (int m = 0; m < 16; m += 2) {
@@ -230,14 +227,14 @@ __device__ __forceinline__ void chol64_blocked(double* S, double* rdiag, double*
   __syncthreads();
 }
 
-// V_b = L_bb^-1 for the four diagonal 16-blocks of L (S, pitch LP, rdiag =
-// 1 / L_ii): wave b forward-substitutes block b, four lanes per column
-// splitting each row's dot product by k mod 4 and combining it by two DPP quad
-// exchanges.  V_b goes to V[(16 b + i) * PV + (DIAG ? 16 b : 0) + j].
+// V = L_bb^-1 for diagonal 16-block b of L (S, pitch LP), by the calling
+// wave: forward substitution with four lanes per column splitting each row's
+// dot product by k mod 4 and combining it by two DPP quad exchanges.  V goes to
+// V[(16 b + i) * PV + (DIAG ? 16 b : 0) + j].
 template <int PV, bool DIAG>
-__device__ __forceinline__ void inv16_diag(const double* S, const double* rdiag, double* V) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int jl = lane >> 2, r = lane & 3, b0 = 16 * wv;
+__device__ __forceinline__ void inv16_block(const double* S, int blk, double* V) {
+  const int lane = threadIdx.x & 63;
+  const int jl = lane >> 2, r = lane & 3, b0 = 16 * blk;
   // this lane's operands L[b0 + i][b0 + 4t + r] (k < i), loaded up front
   double lv[16][4];
 #pragma unroll
@@ -249,7 +246,7 @@ __device__ __forceinline__ void inv16_diag(const double* S, const double* rdiag,
     }
   double rd[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) rd[i] = rdiag[b0 + i];
+  for (int i = 0; i < 16; ++i) rd[i] = 1.0 / S[(b0 + i) * LP + b0 + i];
   double x[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
@@ -265,13 +262,19 @@ __device__ __forceinline__ void inv16_diag(const double* S, const double* rdiag,
   for (int t = 0; t < 4; ++t) V[(b0 + 4 * t + r) * PV + (DIAG ? b0 : 0) + jl] = x[t];
 }
 
+// the four diagonal 16-blocks, one per wave
+template <int PV, bool DIAG>
+__device__ __forceinline__ void inv16_diag(const double* S, double* V) {
+  inv16_block<PV, DIAG>(S, threadIdx.x >> 6, V);
+}
+
 // W = L^-1 (pitch LP) for the lower-triangular L in S, with rdiag = 1 / L_ii:
 // the diagonal 16-blocks by inv16_diag, the off-diagonal blocks by recursive
 // doubling on FP64 MFMA:  W10 = -V1 L10 V0,  W32 = -V3 L32 V2, then
 // W[23][01] = -W[23][23] (L[23][01] W[01][01]).  All 256 threads call it.
 __device__ __forceinline__ void trinv64_blocked(const double* S, const double* rdiag, double* Wl) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  inv16_diag<LP, true>(S, rdiag, Wl);
+  inv16_diag<LP, true>(S, Wl);
   {
     // zero this wave's block row right of its diagonal block
     const int b0 = 16 * wv;
@@ -363,20 +366,21 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
       for (int q = 0; q < 4; ++q)
         dacc[u][q] = Ajj[(int64_t)(16 * dib[u] + g + 4 * q) * lda + 16 * djb[u] + l16];
   }
-  v4d tacc[4];
-  if (own)
+  // T blocks tb = (wv - 1) + 3u of waves 1-3 (row block tb / 4, column block
+  // tb % 4), accumulated while wave 0 factors the diagonal block
+  v4d tacc[6];
+  if (own && wv > 0)
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb)
+    for (int u = 0; u < 6; ++u) {
+      const int tb = wv - 1 + 3 * u;
+      if (tb < 16)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        tacc[nb][q] = Abj[(int64_t)(16 * wv + g + 4 * q) * lda + 16 * nb + l16];
+        for (int q = 0; q < 4; ++q)
+          tacc[u][q] = Abj[(int64_t)(16 * (tb >> 2) + g + 4 * q) * lda + 16 * (tb & 3) + l16];
+    }
   if (j > 0) {
     stage64(Ajj - LEAF, lda, Bf);  // L_j,j-1
     __syncthreads();
-    double qv[16];
-    if (own)
-#pragma unroll
-      for (int s = 0; s < 16; ++s) qv[s] = -Q[(int64_t)(16 * wv + l16) * lda + 4 * s + g];
 #pragma unroll
     for (int u = 0; u < 3; ++u)
       if (wv + 4 * u < 10)
@@ -385,23 +389,31 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
           dacc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
               -Bf[(16 * dib[u] + l16) * LP + 4 * s + g], Bf[(16 * djb[u] + l16) * LP + 4 * s + g],
               dacc[u], 0, 0, 0);
-    if (own)
-#pragma unroll
-      for (int s = 0; s < 16; ++s)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
-          tacc[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              qv[s], Bf[(16 * nb + l16) * LP + 4 * s + g], tacc[nb], 0, 0, 0);
-    __syncthreads();
   }
 #pragma unroll
   for (int u = 0; u < 3; ++u)
     if (wv + 4 * u < 10) st16(S, 16 * dib[u], 16 * djb[u], dacc[u]);
-  if (own)
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) st16(Bf, 16 * wv, 16 * nb, tacc[nb]);
   __syncthreads();
-  chol64_blocked(S, rdiag, colv, flags, flags + 1);
+  // while wave 0 factors panel p: T -= L_b,j-1 L_j,j-1^T over a quarter of K,
+  // and wave p (p >= 1) inverts the diagonal block p - 1 finished before
+  auto bg = [&](int p) {
+    if (!own) return;
+    if (j > 0) {
+#pragma unroll
+      for (int u = 0; u < 6; ++u) {
+        const int tb = wv - 1 + 3 * u;
+        if (tb >= 16) continue;
+        const int rb = tb >> 2, cb = tb & 3;
+#pragma unroll
+        for (int s = 4 * p; s < 4 * p + 4; ++s)
+          tacc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              -Q[(int64_t)(16 * rb + l16) * lda + 4 * s + g], Bf[(16 * cb + l16) * LP + 4 * s + g],
+              tacc[u], 0, 0, 0);
+      }
+    }
+    if (p >= 1 && wv == p) inv16_block<VP, false>(S, p - 1, Vs);
+  };
+  chol64_blocked(S, rdiag, colv, flags, flags + 1, bg);
   if (flags[0] >= 0) {
     if (!own && tid == 0) {
       a.info[0] = j * LEAF + flags[0] + 1;
@@ -416,7 +428,15 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
     }
     return;
   }
-  inv16_diag<VP, false>(S, rdiag, Vs);
+  if (wv > 0) {
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int tb = wv - 1 + 3 * u;
+      if (tb < 16) st16(Bf, 16 * (tb >> 2), 16 * (tb & 3), tacc[u]);
+    }
+  } else {
+    inv16_block<VP, false>(S, 3, Vs);
+  }
   __syncthreads();
   const int r0 = 16 * wv;
 #pragma unroll

@@ -5,10 +5,10 @@
 //   d log p / d theta = 1/2 sum_ij W_ij dK_ij/dtheta,   W = a a^T - K^-1
 //
 // K^-1 = X^T X with X = L^-1 is never stored: mll_trace_kernel forms each
-// lower 128 x 128 tile of it on the FP64 MFMA main loop (d_mainloop, both
+// lower 128 x 64 tile of it on the FP64 MFMA main loop (d_mainloop, both
 // operands read K-major straight from X), the K range of a tile,
-// [128 tm, n_pad), cut into chunks of KC so that every task costs about the
-// same.  Each task's epilogue rebuilds k and dk/dtheta for the tile's point
+// [max(128 tm, 64 tn), n_pad), cut into chunks of KC so that every task costs
+// about the same.  Each task's epilogue rebuilds k and dk/dtheta for the tile's point
 // pairs from the scaled inputs staged in LDS and folds -P_ij dK_ij/dtheta
 // (plus a_i a_j dK_ij/dtheta for the tile's first chunk; the sum is linear in
 // K^-1) into per-worker sums, so the n^2 x (d + 2) derivative tensor is never
@@ -40,30 +40,41 @@ struct TraceArgs {
   double s;
 };
 
-// Tasks of chunk c cover the tiles tm < MT - c CB (those whose K range
-// reaches chunk c), lower tiles in decode_tile's C_LOWER order.
+// Tiles are 128 rows x 64 columns; tile (tm, tn) holds lower elements when
+// tn <= 2 tm + 1 and its K range starts at max(128 tm, 64 tn).  Tasks of chunk
+// c cover the tile rows tm < MT - c CB (those whose K range reaches chunk c),
+// T (T + 1) tiles for T such rows, enumerated row by row.
+__host__ __device__ inline int trace_tasks(int MT, int CB) {
+  int total = 0;
+  for (int c = 0; MT - c * CB > 0; ++c) {
+    const int T = MT - c * CB;
+    total += T * (T + 1);
+  }
+  return total;
+}
 __device__ __forceinline__ bool trace_task(int t, int MT, int CB, int& tm, int& tn, int& c) {
   for (c = 0;; ++c) {
     const int T = MT - c * CB;
     if (T <= 0) return false;
-    const int cnt = T * (T + 1) / 2;
+    const int cnt = T * (T + 1);
     if (t < cnt) break;
     t -= cnt;
   }
-  const int T = MT - c * CB;
-  decode_tile(t, BTA_GEMM_C_LOWER, T, T, tm, tn);
+  int r = (int)((__builtin_sqrt(4.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) <= t) ++r;
+  while (r * (r + 1) > t) --r;
+  tm = r;
+  tn = t - r * (r + 1);
   return true;
 }
 
 template <int MD, int KIND>
-__global__ __launch_bounds__(256, 1) void mll_trace_kernel(TraceArgs a) {
-  __shared__ __attribute__((aligned(1024))) unsigned char lds[DCfg<128>::STAGES * DCfg<128>::STAGE];
-  static_assert(2 * TR_B * MD * 8 <= DCfg<128>::STAGES * DCfg<128>::STAGE, "x staging");
-  int total = 0;
-  for (int c = 0; a.MT - c * a.CB > 0; ++c) {
-    const int T = a.MT - c * a.CB;
-    total += T * (T + 1) / 2;
-  }
+__global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(TraceArgs a) {
+  constexpr int TN = 64;  // tile columns
+  constexpr int RING = DCfg<TN>::STAGES * DCfg<TN>::STAGE;
+  constexpr int STG = (TR_B + TN) * MD * 8;
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[RING > STG ? RING : STG];
+  const int total = trace_tasks(a.MT, a.CB);
   const int W = gridDim.x, w = blockIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
@@ -76,26 +87,25 @@ __global__ __launch_bounds__(256, 1) void mll_trace_kernel(TraceArgs a) {
     const int t = h2_tile_of(jj, w, W);
     int tm, tn, c;
     if (t >= total || !trace_task(t, a.MT, a.CB, tm, tn, c)) continue;
-    const int kb = tm * TR_B + c * TR_KC;
+    const int kb = max(tm * TR_B, tn * TN) + c * TR_KC;
     const int ke = min(a.n_pad, kb + TR_KC);
-    v4d acc[4][4];
-    d_mainloop<128, true, true>(a.X, a.ld, a.X, a.ld, tm * TR_B, tn * TR_B, a.n_pad, a.n_pad, kb,
-                                ke, lds, acc);
-    // the ring is free: stage the tile's scaled rows and columns
+    v4d acc[4][2];
+    d_mainloop<TN, true, true>(a.X, a.ld, a.X, a.ld, tm * TR_B, tn * TN, a.n_pad, a.n_pad, kb, ke,
+                               lds, acc);
+    // the ring is free: stage the tile's scaled rows and columns (xc follows xr)
     double* xr = (double*)lds;
     double* xc = xr + TR_B * MD;
-    for (int e = threadIdx.x; e < TR_B * MD; e += 256) {
+    for (int e = threadIdx.x; e < (TR_B + TN) * MD; e += 256) {
       const int r = e / MD, j = e % MD;
-      const int gi = tm * TR_B + r, gj = tn * TR_B + r;
+      const int gi = r < TR_B ? tm * TR_B + r : tn * TN + r - TR_B;
       xr[e] = (j < d && gi < a.n) ? a.xs[(int64_t)gi * d + j] : 0.0;
-      xc[e] = (j < d && gj < a.n) ? a.xs[(int64_t)gj * d + j] : 0.0;
     }
     __syncthreads();
     const bool first = c == 0;
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-      const int cl = wx * 64 + ni * 16 + l16;
-      const int gj = tn * TR_B + cl;
+    for (int ni = 0; ni < 2; ++ni) {
+      const int cl = wx * 32 + ni * 16 + l16;
+      const int gj = tn * TN + cl;
       const bool jok = gj < a.n;
       const double aj = jok ? a.alpha[gj] : 0.0;
       double xj[MD];
@@ -114,11 +124,10 @@ __global__ __launch_bounds__(256, 1) void mll_trace_kernel(TraceArgs a) {
           const bool ok = jok && gi < a.n && gi >= gj;
           const double wij = (first ? a.alpha[min(gi, a.n_pad - 1)] * aj : 0.0) - pij;
           const double f = ok ? (gi == gj ? 1.0 : 2.0) : 0.0;
-          double r2 = 0.0, xi[MD];
+          double r2 = 0.0;
 #pragma unroll
           for (int j = 0; j < MD; ++j) {
-            xi[j] = xr[rl * MD + j];
-            const double df = xi[j] - xj[j];
+            const double df = xr[rl * MD + j] - xj[j];
             r2 = fma(df, df, r2);
           }
           double kv, h;
@@ -138,7 +147,7 @@ __global__ __launch_bounds__(256, 1) void mll_trace_kernel(TraceArgs a) {
           const double fwh = fw * h;
 #pragma unroll
           for (int j = 0; j < MD; ++j) {
-            const double df = xi[j] - xj[j];
+            const double df = xr[rl * MD + j] - xj[j];
             acc_ls[j] = fma(fwh, df * df, acc_ls[j]);
           }
         }
@@ -370,11 +379,7 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   ta.MT = (int)(np / TR_B);
   ta.CB = TR_KC / TR_B;
   ta.s = s;
-  int tasks = 0;
-  for (int cc = 0; ta.MT - cc * ta.CB > 0; ++cc) {
-    const int T = ta.MT - cc * ta.CB;
-    tasks += T * (T + 1) / 2;
-  }
+  const int tasks = trace_tasks(ta.MT, ta.CB);
   const int W = std::min(w.W, tasks);
   if (d <= 16)
     launch_trace<16>(kind, ta, W, st);
