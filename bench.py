@@ -42,9 +42,9 @@ def _timed(fn, steps, warmup):
 
 
 def build_headline(precision: str, n=2048, d=12, b=512, q=4, S=512):
+    """The product acquisition of the headline workload (inputs on the GPU)."""
     import botorch_amd as B
     from botorch_amd import workloads
-    from oracle import gp_oracle as O
 
     w = workloads.gp_headline(n=n, d=d, b=b, q=q, S=S)
     dev = "cuda"
@@ -55,14 +55,22 @@ def build_headline(precision: str, n=2048, d=12, b=512, q=4, S=512):
     lik = B.GaussianLikelihood()
     lik.noise = w["noise"]
     gp = B.SingleTaskGP(w["X"].to(dev), w["Y"].to(dev), likelihood=lik, covar_module=covar)
-    orc = O.OracleGP(w["X"].to(dev), w["Y"].to(dev), O.MATERN52, w["lengthscale"].to(dev), 1.0,
-                     w["noise"])
-    Xb, _ = O.prune_inferior_points(orc, w["X"].to(dev), num_samples=2048, seed=123)
+    with torch.no_grad():
+        Xb = B.prune_inferior_points(
+            gp, w["X"].to(dev), sampler=B.SobolQMCNormalSampler(torch.Size([2048]), seed=123))
     sampler = B.SobolQMCNormalSampler(torch.Size([S]), seed=0)
     acq = B.qLogNoisyExpectedImprovement(gp, Xb, sampler=sampler, prune_baseline=False,
                                          precision=precision)
-    ref = O.OracleQLogNEI(orc, Xb, num_samples=S, seed=0)
-    return acq, ref, w, orc, Xb
+    return acq, w, Xb
+
+
+def build_comparator(w, Xb, S, dev="cuda"):
+    """FP64 oracle on the same inputs: the accuracy checker and the plain FP64
+    PyTorch baseline leg."""
+    from oracle import gp_oracle as O
+    orc = O.OracleGP(w["X"].to(dev), w["Y"].to(dev), O.MATERN52, w["lengthscale"].to(dev), 1.0,
+                     w["noise"])
+    return O.OracleQLogNEI(orc, Xb, num_samples=S, seed=0)
 
 
 def headline_flops(n, d, b, q, n_b, S):
@@ -122,6 +130,40 @@ def gemm_fwd_roofline(acq, X, steps, precision):
     return avg_ms, flops
 
 
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "gemm_h2_kernel_qlognei_n2048_b512_q4_pmc.txt")
+
+
+def committed_traffic():
+    """HBM bytes per launch of the forward contraction, from the committed
+    FETCH_SIZE / WRITE_SIZE passes (line ``traffic_bytes_per_launch = N``)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(float(line.split("=")[1]))
+    except OSError:
+        pass
+    return None
+
+
+def roofline_only(reps=10):
+    """Counter passes: a plain copy of known size, then the forward contraction
+    of the mixed step, each launched ``reps`` times after the set-up work."""
+    acq, w, _ = build_headline("mixed")
+    X = w["Xq"].to("cuda")
+    acq._launch(X, True)
+    nbytes = 256 << 20
+    src = torch.rand(nbytes // 4, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    gemm_fwd_roofline(acq, X, reps, "mixed")
+    torch.cuda.synchronize()
+    log(f"copy kernel: {nbytes} bytes read + {nbytes} bytes written per launch, {reps} launches")
+
+
 def cpu_baseline_headline(w, Xb, budget_s=15.0):
     """FP64 oracle (PyTorch CPU, reference op sequence) on a bounded sample of
     the headline workload: value+grad over a slice of the restarts."""
@@ -164,6 +206,8 @@ def main():
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-mll", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="launch only a copy and the forward contraction (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
@@ -172,6 +216,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
+    if args.roofline_only:
+        roofline_only()
+        return
     from botorch_amd import _lib
     from oracle import gp_oracle as O
 
@@ -179,7 +226,7 @@ def main():
     out = {}
     # ---------------- mixed-precision HIP path (the measured value) --------------
     log("building headline model / acquisition (mixed)")
-    acq_m, ref, w, orc, Xb = build_headline("mixed", n, d, b, q, S)
+    acq_m, w, Xb = build_headline("mixed", n, d, b, q, S)
     X = w["Xq"].to("cuda")
     n_b = Xb.shape[0]
 
@@ -204,6 +251,7 @@ def main():
     acq_f.check_status(st_f)
     log(f"mixed_f32: {t_f32 * 1e3:.3f} ms/step")
     # ---------------- accuracy vs the FP64 oracle (same inputs, on device) ---------
+    ref = build_comparator(w, Xb, S)
     v_ref, g_ref = ref.value_and_grad(X)
 
     def rel(a, r):
@@ -224,7 +272,7 @@ def main():
     # split-FP16: each algorithmic FMA costs three FP16 MFMA FMAs
     peak = PEAK_TFLOPS["f16"] / 3.0
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 1),
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": committed_traffic(),
             "kernel": "gemm_h2_kernel V = K_x L^-T (split-FP16 planes, 3 products)",
             "peak_basis": "dense FP16 MFMA 2500 TFLOP/s / 3 products",
             "avg_launch_ms": round(gemm_ms, 5), "flops_per_launch": gemm_flops}
