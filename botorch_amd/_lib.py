@@ -116,6 +116,8 @@ SIGNATURES = {
                                           _f64, _f64, _i32, ctypes.POINTER(_f64), _p, _sz, _p]),
 }
 
+ABI_VERSION = 4  # BTA_ABI_VERSION of include/botorch_amd.h
+
 _lock = threading.Lock()
 _lib = None
 
@@ -133,6 +135,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                 fn = getattr(lib, name)
                 fn.restype = res
                 fn.argtypes = args
+            if lib.bta_abi_version() != ABI_VERSION:
+                raise RuntimeError(f"{path} has C ABI {lib.bta_abi_version()}, this binding "
+                                   f"expects {ABI_VERSION}: rebuild the library.")
             _lib = lib
     return _lib
 

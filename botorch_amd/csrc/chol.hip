@@ -2,13 +2,18 @@
 // triangular inverse, and a batched small Cholesky for q x q / n_b x n_b
 // blocks.
 //
-// bta_potrf is right-looking with 64-column panels:
-//   leaf   : one 256-thread block factors the 64 x 64 diagonal block in LDS and
-//            inverts it (W = L_jj^-1, written into Linv's diagonal block);
-//   panel  : L_ij = A_ij W^T for the blocks below - one MFMA GEMM (B lower);
-//   update : A_ik -= L_ij L_kj^T on the lower trailing tiles - one MFMA GEMM.
-// The trailing update dominates the flops and runs on the MFMA GEMM; the leaf
-// is latency-bound.
+// bta_potrf is right-looking with 64-column block columns, one launch per
+// block column j (potrf_step_kernel), the update of column j-1 fused with the
+// factorisation of column j:
+//   strip : one workgroup per row block b >= j applies column j-1's update to
+//           the diagonal block and to block b, factors the diagonal block in
+//           LDS (FP64 MFMA on 16-column panels) and solves L_bj;
+//   bulk  : the other workgroups apply column j-1's rank-64 update to the
+//           trailing lower 128 x 128 tiles right of column j (direct-to-LDS
+//           FP64 MFMA main loop of the GEMM).
+// Early steps are bound by the bulk update, late steps by the latency of the
+// diagonal factorisation.  trinv_diag_kernel then inverts every 64 x 64
+// diagonal block into Linv.
 //
 // bta_trtri_lower inverts L bottom-up by recursive doubling: with the 64x64
 // diagonal inverses in place, each level joins pairs of inverted diagonal
@@ -320,7 +325,8 @@ __device__ __forceinline__ void trinv64_blocked(const double* S, const double* r
 // L_jj; the others solve their block, L_bj L_jj^T = A_bj - L_b,j-1 L_j,j-1^T,
 // by blocked forward substitution: wave w takes rows [16w, 16w + 16) and the
 // four 16-column blocks in turn, X_k = (T_k - sum_{m<k} X_m L_km^T) V_k^T
-// with V_k = L_kk^-1 from inv16_diag, so no wave waits for another.  The
+// with V_k = L_kk^-1 (inv16_block), so no wave waits for another.  Waves 1-3
+// form the blocks T_k and V_0..V_2 while wave 0 factors the panels.  The
 // remaining workgroups apply column j-1's update to the trailing lower tiles
 // right of column j (the strip covers column j): A_bc -= L_b,j-1 L_c,j-1^T on
 // 128 x 128 FP64 MFMA tiles.

@@ -1,9 +1,16 @@
 // qLogNEI forward + backward, fused per restart.
 //
-// The posterior contraction V = K(X, X_train) L^-T comes from the MFMA GEMM
-// (FP64, or FP32 with FP64 segment sums); the 1 + n_b dense products
-// P = K(X, X_train) [alpha | K^-1 K(X_train, X_b)] are formed in FP64.  Everything else for one restart b is small and
-// done by one 256-thread workgroup in LDS (qlognei_mc_kernel):
+// One step (R = b q query rows, n training points):
+//   qstep_covar   xs, K_x = s k(xs, X_train) and the FP64 partial products
+//                 P = K_x [alpha | A]  (A = K^-1 K(X_train, X_b))
+//   GEMM          V = K_x L^-T         (FP64, FP32, or split-FP16 planes)
+//   GEMM          U = V L^-1 = K_x K^-1 (gradient only; independent of the
+//                 MC stage, so it runs before it)
+//   qlognei_mc    per restart, below
+//   qstep_grad    dX from U, gq, dext and dK/dxs (qstep.hip)
+//
+// Everything between the products and the input gradient is small and done
+// by one 256-thread workgroup per restart b in LDS (qlognei_mc_kernel):
 //
 //   Sigma_qq = s^2 (K(X,X) - V V^T)        Sigma_qb = s^2 (K(X,X_b) - K_x A)
 //   mu       = y_mean + s (c + K_x alpha)                       (s = y_std)
@@ -15,10 +22,10 @@
 // followed by the closed-form adjoint of every step (softmax weights of the
 // logmeanexp, the fat-max Pareto derivative with the amax tie split, the
 // fat-softplus derivative, Z-contractions, the q x q Cholesky backward, the
-// triangular solve with Lb), producing dV[r] = -2 s^2 (gA V)_r for the
-// adjoint GEMM with L^-1, dext[r] = [s dmu_r | -s^2 dSigma_qb[r]] for the FP64
-// products with [alpha | A], and the direct input gradient from K(X,X) and
-// K(X,X_b).  References: utils/low_rank.py:84-172,
+// triangular solve with Lb).  The adjoint of V V^T enters as the q x q block
+// gq = -2 s^2 gA of each restart (dK_x = gq U for its rows), the adjoint of P
+// as dext[r] = [s dmu_r | -s^2 dSigma_qb[r]], and the direct input gradient
+// from K(X,X) and K(X,X_b) as dxs_extra.  References: utils/low_rank.py:84-172,
 // acquisition/logei.py:376-391,688-715, utils/safe_math.py:146-191,298-355,
 // acquisition/monte_carlo.py:268-290.
 #include "bta_common.h"

@@ -153,8 +153,10 @@ size_t bta_potrf_workspace_size(int64_t n_pad);
  * try, BTA_ERR_NAN if A holds NaN.  Also writes the inverses of the 64x64
  * diagonal blocks of L into the diagonal blocks of Linv (ld ldinv; may be NULL)
  * and zeroes the strict upper triangle of A.  *jitter_used (host) receives the
- * jitter that was added (0 if none).  SYNCHRONISES on `stream` (reads info),
- * as the reference does on any(info). */
+ * jitter that was added (0 if none).  A must be 16-byte aligned with an even
+ * lda (BTA_ERR_INVALID otherwise: the trailing updates load A directly into
+ * LDS 16 bytes at a time).  SYNCHRONISES on `stream` (reads info), as the
+ * reference does on any(info). */
 int bta_potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv,
               int32_t max_tries, double jitter0, double* jitter_used, void* ws,
               size_t ws_bytes, bta_stream_t stream);

@@ -214,3 +214,38 @@ def test_mll_value_and_adjoint(kind):
     assert got[3] == pytest.approx(cr.grad.item(), rel=1e-7, abs=1e-9)
     torch.testing.assert_close(torch.tensor(got[4:], dtype=torch.float64), lsr.grad, rtol=1e-7,
                                atol=1e-9)
+
+
+def test_mll_full_size_n4096_d16():
+    """The MLL workload at its measured size (n = 4096, d = 16, GP-prior draw
+    with l = 0.4 plus noise 0.05^2) against the oracle's autograd on the GPU."""
+    _say("mll n=4096 d=16 (full size)")
+    import ctypes
+    from botorch_amd import _lib
+    w = workloads.gp_mll(n=4096, d=16)
+    X = w["X"].to(DEV)
+    f = workloads.prior_draw(X, w["z"].to(DEV), w["lengthscale"])
+    y = f + w["noise_sd"] * w["eps"].to(DEV)
+    y = ((y - y.mean()) / y.std()).contiguous()
+    n, d = X.shape
+    ls = 0.3 + 0.2 * torch.arange(d, dtype=torch.float64) / d
+    os_, noise, c = 1.2, 0.05**2, 0.05
+    lsr = ls.to(DEV).requires_grad_(True)
+    osr = torch.tensor(os_, dtype=torch.float64, device=DEV, requires_grad=True)
+    nr = torch.tensor(noise, dtype=torch.float64, device=DEV, requires_grad=True)
+    cr = torch.tensor(c, dtype=torch.float64, device=DEV, requires_grad=True)
+    ll = O.exact_mll(O.MATERN52, X, y, lsr, osr, nr, cr) * n
+    ll.backward()
+    L = _lib.lib()
+    ws = _lib.workspace(L.bta_mll_workspace_size(n, d), DEV)
+    out = (ctypes.c_double * (4 + d))()
+    lsh = (ctypes.c_double * d)(*ls.tolist())
+    for _ in range(2):  # the second call reuses the workspace left by the first
+        _lib.check(L.bta_mll_value_grad(O.MATERN52, _lib.ptr(X.contiguous()), _lib.ptr(y), n, d,
+                                        lsh, os_, noise, c, 6, out, _lib.ptr(ws), ws.numel(),
+                                        _lib.stream()))
+        got = torch.tensor(list(out), dtype=torch.float64)
+        ref = torch.cat([torch.stack([ll.detach(), osr.grad, nr.grad, cr.grad]).cpu(),
+                         lsr.grad.cpu()])
+        assert abs(got[0] - ref[0]) <= 1e-10 * abs(ref[0])
+        assert _rel(got[1:], ref[1:]) < 1e-8
