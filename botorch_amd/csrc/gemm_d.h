@@ -36,6 +36,10 @@ __device__ __forceinline__ void d_wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
+// v from the first active lane: a value the compiler then treats as scalar.
+// (A non-template function: the builtin exists only in the device pass.)
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // LDS fragment reads as inline asm.  The compiler then sees no LDS read after
 // the direct loads, so it adds no s_waitcnt vmcnt(0) in front of every chunk's
 // fragment reads (which would drain the whole ring each iteration); the
@@ -122,95 +126,114 @@ __device__ __forceinline__ void d_mainloop(const double* __restrict__ A, int64_t
 #pragma unroll
     for (int ni = 0; ni < C::FN; ++ni) acc[mi][ni] = (v4d)(0.0);
 
+  // Each wave load j of a chunk reads one 1 KiB piece: its global address at
+  // chunk 0, the per-chunk advance (D_BK columns, or D_BK rows of a K-major
+  // operand) and its LDS offset are fixed, so the loop only adds ci * step.
+  // The wave index is made scalar so the A/B choice is a scalar branch.
+  const int wvs = wave_uniform(wv);
+  const double* gsrc[C::LOADS_W];
+  int64_t gstep[C::LOADS_W];
+  int goff[C::LOADS_W];
+#pragma unroll
+  for (int j = 0; j < C::LOADS_W; ++j) {
+    const int gi = wvs * C::LOADS_W + j;
+    if (gi < C::LA) {
+      goff[j] = gi * 1024;
+      if (AKM) {  // stage rows are k, 128 doubles each: one k row per wave load
+        int m = row0 + lane * 2;
+        if (m >= M) m -= 64;
+        gsrc[j] = A + (int64_t)(kb + gi) * lda + m;
+        gstep[j] = (int64_t)D_BK * lda;
+      } else {
+        int r = row0 + gi * 16 + lrow;
+        if (r >= M) r -= 64;
+        gsrc[j] = A + (int64_t)r * lda + kb + lchunk * 2;
+        gstep[j] = D_BK;
+      }
+    } else {
+      const int gb = gi - C::LA;
+      goff[j] = C::PA + gb * 1024;
+      if (BKN) {  // stage rows are k, BN doubles each
+        const int e = gb * 128 + lane * 2;
+        const int kr = e / BN;
+        int n = col0 + e % BN;
+        if (n >= N) n -= 64;
+        gsrc[j] = B + (int64_t)(kb + kr) * ldb + n;
+        gstep[j] = (int64_t)D_BK * ldb;
+      } else {
+        int r = col0 + gb * 16 + lrow;
+        if (r >= N) r -= 64;
+        gsrc[j] = B + (int64_t)r * ldb + kb + lchunk * 2;
+        gstep[j] = D_BK;
+      }
+    }
+  }
   auto issue = [&](int ci) {
-    const int k0 = kb + ci * D_BK;
     unsigned char* st = lds + (ci % S) * C::STAGE;
 #pragma unroll
-    for (int j = 0; j < C::LOADS_W; ++j) {
-      const int gi = wv * C::LOADS_W + j;  // wave-uniform
-      const double* src;
-      int off;
-      if (gi < C::LA) {
-        off = gi * 1024;
-        if (AKM) {  // stage rows are k, 128 doubles each: one k row per wave load
-          int m = row0 + lane * 2;
-          if (m >= M) m -= 64;
-          src = A + (int64_t)(k0 + gi) * lda + m;
-        } else {
-          int r = row0 + gi * 16 + lrow;
-          if (r >= M) r -= 64;
-          src = A + (int64_t)r * lda + k0 + lchunk * 2;
-        }
-      } else {
-        const int gb = gi - C::LA;
-        off = C::PA + gb * 1024;
-        if (BKN) {  // stage rows are k, BN doubles each
-          const int e = gb * 128 + lane * 2;
-          const int kr = e / BN;
-          int n = col0 + e % BN;
-          if (n >= N) n -= 64;
-          src = B + (int64_t)(k0 + kr) * ldb + n;
-        } else {
-          int r = col0 + gb * 16 + lrow;
-          if (r >= N) r -= 64;
-          src = B + (int64_t)r * ldb + k0 + lchunk * 2;
-        }
-      }
-      __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(st + off), 16, 0, 0);
+    for (int j = 0; j < C::LOADS_W; ++j)
+    {
+      // (a named pointer: the builtin rejects the expression in the host pass)
+      const double* src = gsrc[j] + ci * gstep[j];
+      __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(st + goff[j]), 16, 0, 0);
     }
   };
 
-#pragma unroll
-  for (int ci = 0; ci < S - 1; ++ci)
-    if (ci < nk) issue(ci);
   const int foff = l16 * 64 + ((gq ^ d_swizzle(l16)) << 4);
-  for (int it = 0; it < nk; ++it) {
-    const int later = min(S - 2, nk - 1 - it);
+  // fragment reads of chunk ci into (a, b), not waited for
+  auto read = [&](int ci, double (&a)[C::FM][2], double (&b)[C::FN][2]) {
+    const unsigned char* cur = lds + (ci % S) * C::STAGE;
+    if (AKM) {
+      const uint32_t a0 = lds_addr((const double*)cur + (wy * 64 + l16) + 2 * gq * D_BM);
+      frag_mn<C::FM, D_BM>(a0, a);
+    } else {
+      const uint32_t a0 = lds_addr(cur + wy * 64 * 64 + foff);
+      frag_nk<C::FM>(a0, a);
+    }
+    if (BKN) {
+      const uint32_t b0 = lds_addr((const double*)(cur + C::PA) + (wx * C::WN + l16) + 2 * gq * BN);
+      frag_mn<C::FN, BN>(b0, b);
+    } else {
+      const uint32_t b0 = lds_addr(cur + C::PA + wx * C::WN * 64 + foff);
+      frag_nk<C::FN>(b0, b);
+    }
+  };
+  auto wait_ties = [&](double (&a)[C::FM][2], double (&b)[C::FN][2]) {
+    lds_frag_wait();
+#pragma unroll
+    for (int f = 0; f < C::FM; ++f) {
+      lds_tie(a[f][0]);
+      lds_tie(a[f][1]);
+    }
+#pragma unroll
+    for (int f = 0; f < C::FN; ++f) {
+      lds_tie(b[f][0]);
+      lds_tie(b[f][1]);
+    }
+  };
+  // every wave's loads of chunk ci landed, and every wave finished the reads
+  // it issued before: a wait for this wave's own loads (all but `later`
+  // chunks), then a bare s_barrier (the fence of __syncthreads would wait for
+  // the loads still in flight).  sched_barrier(0): the compiler moves nothing
+  // across, in particular no LDS read above the wait and the barrier (another
+  // wave's direct loads fill part of the stage, and only the barrier orders
+  // them).
+  auto arrive = [&](int later) {
     if (later >= 2)
       d_wait_vmcnt<2 * C::LOADS_W>();
     else if (later == 1)
       d_wait_vmcnt<C::LOADS_W>();
     else
       d_wait_vmcnt<0>();
-    // every wave's loads of chunk it landed and every wave is done with chunk
-    // it - 1 (whose stage the next issue overwrites).  A bare s_barrier: the
-    // fence of __syncthreads would wait for the loads still in flight.
-    // sched_barrier(0): the compiler moves nothing across, in particular no
-    // LDS read of this chunk above the wait and the barrier (another wave's
-    // direct loads fill part of the stage, and only the barrier orders them)
     __builtin_amdgcn_sched_barrier(0);
     __asm__ __volatile__("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     __asm__ __volatile__("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    if (it + S - 1 < nk) issue(it + S - 1);
-    const unsigned char* cur = lds + (it % S) * C::STAGE;
-    double af[C::FM][2], bf[C::FN][2];
-    if (AKM) {
-      const uint32_t a0 = lds_addr((const double*)cur + (wy * 64 + l16) + 2 * gq * D_BM);
-      frag_mn<C::FM, D_BM>(a0, af);
-    } else {
-      const uint32_t a0 = lds_addr(cur + wy * 64 * 64 + foff);
-      frag_nk<C::FM>(a0, af);
-    }
-    if (BKN) {
-      const uint32_t b0 = lds_addr((const double*)(cur + C::PA) + (wx * C::WN + l16) + 2 * gq * BN);
-      frag_mn<C::FN, BN>(b0, bf);
-    } else {
-      const uint32_t b0 = lds_addr(cur + C::PA + wx * C::WN * 64 + foff);
-      frag_nk<C::FN>(b0, bf);
-    }
-    lds_frag_wait();
-#pragma unroll
-    for (int f = 0; f < C::FM; ++f) {
-      lds_tie(af[f][0]);
-      lds_tie(af[f][1]);
-    }
-#pragma unroll
-    for (int f = 0; f < C::FN; ++f) {
-      lds_tie(bf[f][0]);
-      lds_tie(bf[f][1]);
-    }
+  };
+  static_assert(S == 4, "the vmcnt levels of arrive() assume 4 stages");
+
+  auto mfma = [&](double (&a)[C::FM][2], double (&b)[C::FN][2]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -218,7 +241,46 @@ __device__ __forceinline__ void d_mainloop(const double* __restrict__ A, int64_t
 #pragma unroll
         for (int ni = 0; ni < C::FN; ++ni)
           acc[mi][ni] =
-              __builtin_amdgcn_mfma_f64_16x16x4f64(af[mi][s], bf[ni][s], acc[mi][ni], 0, 0, 0);
+              __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi][s], b[ni][s], acc[mi][ni], 0, 0, 0);
+  };
+
+#pragma unroll
+  for (int ci = 0; ci < S - 1; ++ci)
+    if (ci < nk) issue(ci);
+  // Chunk it's fragments are read one iteration ahead, so its MFMAs overlap
+  // the reads of chunk it + 1.  Chunks in flight: it + 2 .. it + S (the
+  // issue at iteration it reuses chunk it's stage, whose reads finished at
+  // it - 1).
+  double af[C::FM][2], bf[C::FN][2], an[C::FM][2], bn[C::FN][2];
+  if (nk > 0) {
+    arrive(min(S - 2, nk - 1));
+    if (S - 1 < nk) issue(S - 1);
+    read(0, af, bf);
+    wait_ties(af, bf);
+  }
+  for (int it = 0; it < nk; ++it) {
+    const bool more = it + 1 < nk;
+    if (more) {
+      arrive(min(S - 2, nk - 2 - it));
+      if (it + S < nk) issue(it + S);
+      read(it + 1, an, bn);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(af, bf);
+    __builtin_amdgcn_sched_barrier(0);
+    if (more) {
+      wait_ties(an, bn);
+#pragma unroll
+      for (int f = 0; f < C::FM; ++f) {
+        af[f][0] = an[f][0];
+        af[f][1] = an[f][1];
+      }
+#pragma unroll
+      for (int f = 0; f < C::FN; ++f) {
+        bf[f][0] = bn[f][0];
+        bf[f][1] = bn[f][1];
+      }
+    }
   }
   __syncthreads();
 }
