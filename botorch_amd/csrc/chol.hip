@@ -9,7 +9,7 @@
 //           the diagonal block and to block b, factors the diagonal block in
 //           LDS (FP64 MFMA on 16-column panels) and solves L_bj;
 //   bulk  : the other workgroups apply column j-1's rank-64 update to the
-//           trailing lower 128 x 128 tiles right of column j (direct-to-LDS
+//           trailing lower 128 x 64 tiles right of column j (direct-to-LDS
 //           FP64 MFMA main loop of the GEMM).
 // Early steps are bound by the bulk update, late steps by the latency of the
 // diagonal factorisation.  trinv_diag_kernel then inverts every 64 x 64
@@ -329,7 +329,7 @@ __device__ __forceinline__ void trinv64_blocked(const double* S, const double* r
 // form the blocks T_k and V_0..V_2 while wave 0 factors the panels.  The
 // remaining workgroups apply column j-1's update to the trailing lower tiles
 // right of column j (the strip covers column j): A_bc -= L_b,j-1 L_c,j-1^T on
-// 128 x 128 FP64 MFMA tiles.
+// 128 x 64 FP64 MFMA tiles.
 struct StepArgs {
   double* A;
   int64_t lda;
@@ -461,49 +461,51 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
   for (int t = 0; t < 16; ++t) Abj[(int64_t)(r0 + t) * lda + lane] = Bf[(r0 + t) * LP + lane];
 }
 
+// Rank-64 update of the trailing lower matrix on 128 x 64 tiles (tile row tm
+// holds the 2 tm + 2 tiles tn <= 2 tm + 1).  Every tile costs the same
+// (K = 64), so workers take tiles in a plain stride.  The wave's part of C is
+// loaded before the products: its latency overlaps the first chunks of the
+// main loop instead of following it.
 __device__ __forceinline__ void potrf_bulk(const StepArgs& a, unsigned char* smem) {
   const int j = a.j;
   const int rem = (a.NB - j - 1) * LEAF;
   const int64_t lda = a.lda;
   const double* P = a.A + (int64_t)(j + 1) * LEAF * lda + (int64_t)(j - 1) * LEAF;
   double* Cm = a.A + (int64_t)(j + 1) * LEAF * lda + (int64_t)(j + 1) * LEAF;
-  const int MT = (rem + D_BM - 1) / D_BM, total = MT * (MT + 1) / 2;
+  const int MT = (rem + D_BM - 1) / D_BM, total = MT * (MT + 1);
   const int W = gridDim.x - a.nstrip, w = blockIdx.x - a.nstrip;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
-  for (int jj = 0; jj * W < total; ++jj) {
-    const int t = h2_tile_of(jj, w, W);
-    if (t >= total) continue;
-    int tm, tn;
-    decode_tile(t, BTA_GEMM_C_LOWER, MT, MT, tm, tn);
-    v4d acc[4][4];
-    d_mainloop<128, false, false>(P, lda, P, lda, tm * D_BM, tn * D_BM, rem, rem, 0, LEAF, smem,
-                                  acc);
-    // per 16-row block, all loads before the stores (a store could alias a
-    // later load, which would otherwise serialise every element on memory
-    // latency)
+  for (int t = w; t < total; t += W) {
+    int tm = 0;
+    while ((tm + 1) * (tm + 2) <= t) ++tm;  // tile rows before tm hold tm (tm + 1) tiles
+    const int tn = t - tm * (tm + 1);
+    double cv[4][2][4];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi) {
-      double cv[4][4];
+    for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-          const int col = tn * D_BM + wx * 64 + ni * 16 + l16;
-          cv[ni][q] = (row < rem && col <= row) ? Cm[(int64_t)row * lda + col] : 0.0;
+        for (int ni = 0; ni < 2; ++ni) {
+          const int col = tn * 64 + wx * 32 + ni * 16 + l16;
+          cv[mi][ni][q] = (row < rem && col <= row) ? Cm[(int64_t)row * lda + col] : 0.0;
         }
       }
+    v4d acc[4][2];
+    d_mainloop<64, false, false>(P, lda, P, lda, tm * D_BM, tn * 64, rem, rem, 0, LEAF, smem,
+                                 acc);
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-          const int col = tn * D_BM + wx * 64 + ni * 16 + l16;
-          if (row < rem && col <= row) Cm[(int64_t)row * lda + col] = cv[ni][q] - acc[mi][ni][q];
+        for (int ni = 0; ni < 2; ++ni) {
+          const int col = tn * 64 + wx * 32 + ni * 16 + l16;
+          if (row < rem && col <= row) Cm[(int64_t)row * lda + col] = cv[mi][ni][q] - acc[mi][ni][q];
         }
       }
-    }
   }
 }
 
@@ -659,8 +661,14 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
       sa.nstrip = NB - j;
       int bulk = 0;
       if (j > 0 && NB - j - 1 > 0) {
-        const int MT = (NB - j - 1 + 1) / 2;  // 128-row tiles over the trailing blocks
-        bulk = std::min(MT * (MT + 1) / 2, std::max(2 * num_cus() - sa.nstrip, num_cus()));
+        // The strip workgroups (dispatched first, one per CU) are the step's
+        // critical path once the update is small: if at most two rounds of
+        // tiles fit on the CUs they leave free, keep every strip workgroup
+        // alone on its CU; otherwise fill the second slot of every CU.
+        const int MT = (NB - j - 1 + 1) / 2;  // 128-row tile rows over the trailing blocks
+        const int tiles = MT * (MT + 1), ncu = num_cus();
+        const int w1 = std::max(ncu - sa.nstrip, 1), w2 = std::max(2 * ncu - sa.nstrip, 1);
+        bulk = std::min(tiles, (tiles + w1 - 1) / w1 <= 2 ? w1 : w2);
       }
       hipLaunchKernelGGL(potrf_step_kernel, dim3((unsigned)(sa.nstrip + bulk)), dim3(256), 0, s,
                          sa);
