@@ -136,6 +136,27 @@ __device__ __forceinline__ void stage64(const double* __restrict__ src, int64_t 
   }
 }
 
+// Two 64 x 64 blocks at once (the second only when `two`): all loads, then
+// all LDS stores.
+__device__ __forceinline__ void stage64x2(const double* __restrict__ s1, double* d1,
+                                          const double* __restrict__ s2, double* d2,
+                                          int64_t ld, bool two) {
+  const int tid = threadIdx.x;
+  double v[16], w[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int e = tid + 256 * t;
+    v[t] = s1[(int64_t)(e >> 6) * ld + (e & 63)];
+    if (two) w[t] = s2[(int64_t)(e >> 6) * ld + (e & 63)];
+  }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int e = tid + 256 * t;
+    d1[(e >> 6) * LP + (e & 63)] = v[t];
+    if (two) d2[(e >> 6) * LP + (e & 63)] = w[t];
+  }
+}
+
 // Work the other three waves do while wave 0 factors a panel.
 struct NoBackground {
   __device__ void operator()(int) const {}
@@ -325,8 +346,9 @@ __device__ __forceinline__ void trinv64_blocked(const double* S, const double* r
 // L_jj; the others solve their block, L_bj L_jj^T = A_bj - L_b,j-1 L_j,j-1^T,
 // by blocked forward substitution: wave w takes rows [16w, 16w + 16) and the
 // four 16-column blocks in turn, X_k = (T_k - sum_{m<k} X_m L_km^T) V_k^T
-// with V_k = L_kk^-1 (inv16_block), so no wave waits for another.  Waves 1-3
-// form the blocks T_k and V_0..V_2 while wave 0 factors the panels.  The
+// with V_k = L_kk^-1 (inv16_block), so no wave waits for another.  T is
+// formed before the factorisation; waves 1-3 invert V_0..V_2 while wave 0
+// factors the panels.  The
 // remaining workgroups apply column j-1's update to the trailing lower tiles
 // right of column j (the strip covers column j): A_bc -= L_b,j-1 L_c,j-1^T on
 // 128 x 64 FP64 MFMA tiles.
@@ -344,7 +366,7 @@ constexpr int STEP_LDS = STEP_LDS_STRIP > DCfg<128>::STAGES * DCfg<128>::STAGE
                              : DCfg<128>::STAGES * DCfg<128>::STAGE;
 
 __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* smem) {
-  double* S = (double*)smem;     // diagonal block -> L_jj
+  double* S = (double*)smem;     // L_b,j-1 staging, then the diagonal block -> L_jj
   double* Bf = S + LEAF * LP;    // L_j,j-1 staging, then this block's T -> X
   double* Vs = Bf + LEAF * LP;   // V_k, k = 0..3 (rows 16k.., pitch VP)
   double* rdiag = Vs + LEAF * VP;
@@ -372,20 +394,18 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
       for (int q = 0; q < 4; ++q)
         dacc[u][q] = Ajj[(int64_t)(16 * dib[u] + g + 4 * q) * lda + 16 * djb[u] + l16];
   }
-  // T blocks tb = (wv - 1) + 3u of waves 1-3 (row block tb / 4, column block
-  // tb % 4), accumulated while wave 0 factors the diagonal block
-  v4d tacc[6];
-  if (own && wv > 0)
+  // T = A_bj - L_b,j-1 L_j,j-1^T: wave wv holds column block wv, row blocks
+  // u = 0..3, formed before the factorisation (L_b,j-1 staged in S, which
+  // takes D afterwards)
+  v4d tacc[4];
+  if (own)
 #pragma unroll
-    for (int u = 0; u < 6; ++u) {
-      const int tb = wv - 1 + 3 * u;
-      if (tb < 16)
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          tacc[u][q] = Abj[(int64_t)(16 * (tb >> 2) + g + 4 * q) * lda + 16 * (tb & 3) + l16];
-    }
+      for (int q = 0; q < 4; ++q)
+        tacc[u][q] = Abj[(int64_t)(16 * u + g + 4 * q) * lda + 16 * wv + l16];
   if (j > 0) {
-    stage64(Ajj - LEAF, lda, Bf);  // L_j,j-1
+    stage64x2(Ajj - LEAF, Bf, Q, S, lda, own);  // L_j,j-1 -> Bf, L_b,j-1 -> S
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < 3; ++u)
@@ -395,29 +415,24 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
           dacc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
               -Bf[(16 * dib[u] + l16) * LP + 4 * s + g], Bf[(16 * djb[u] + l16) * LP + 4 * s + g],
               dacc[u], 0, 0, 0);
+    if (own)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int s = 0; s < 16; ++s)
+          tacc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              -S[(16 * u + l16) * LP + 4 * s + g], Bf[(16 * wv + l16) * LP + 4 * s + g], tacc[u],
+              0, 0, 0);
+    __syncthreads();  // S (L_b,j-1) read by every wave
   }
 #pragma unroll
   for (int u = 0; u < 3; ++u)
     if (wv + 4 * u < 10) st16(S, 16 * dib[u], 16 * djb[u], dacc[u]);
   __syncthreads();
-  // while wave 0 factors panel p: T -= L_b,j-1 L_j,j-1^T over a quarter of K,
-  // and wave p (p >= 1) inverts the diagonal block p - 1 finished before
+  // while wave 0 factors panel p, wave p (p >= 1) inverts the diagonal block
+  // p - 1 finished before
   auto bg = [&](int p) {
-    if (!own) return;
-    if (j > 0) {
-#pragma unroll
-      for (int u = 0; u < 6; ++u) {
-        const int tb = wv - 1 + 3 * u;
-        if (tb >= 16) continue;
-        const int rb = tb >> 2, cb = tb & 3;
-#pragma unroll
-        for (int s = 4 * p; s < 4 * p + 4; ++s)
-          tacc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              -Q[(int64_t)(16 * rb + l16) * lda + 4 * s + g], Bf[(16 * cb + l16) * LP + 4 * s + g],
-              tacc[u], 0, 0, 0);
-      }
-    }
-    if (p >= 1 && wv == p) inv16_block<VP, false>(S, p - 1, Vs);
+    if (own && p >= 1 && wv == p) inv16_block<VP, false>(S, p - 1, Vs);
   };
   chol64_blocked(S, rdiag, colv, flags, flags + 1, bg);
   if (flags[0] >= 0) {
@@ -434,15 +449,9 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
     }
     return;
   }
-  if (wv > 0) {
 #pragma unroll
-    for (int u = 0; u < 6; ++u) {
-      const int tb = wv - 1 + 3 * u;
-      if (tb < 16) st16(Bf, 16 * (tb >> 2), 16 * (tb & 3), tacc[u]);
-    }
-  } else {
-    inv16_block<VP, false>(S, 3, Vs);
-  }
+  for (int u = 0; u < 4; ++u) st16(Bf, 16 * u, 16 * wv, tacc[u]);
+  if (wv == 0) inv16_block<VP, false>(S, 3, Vs);
   __syncthreads();
   const int r0 = 16 * wv;
 #pragma unroll
