@@ -83,6 +83,8 @@ __device__ __forceinline__ double quad_perm_d(double v) {
 constexpr int QP_XOR1 = 0xB1;  // quad_perm [1, 0, 3, 2]
 constexpr int QP_XOR2 = 0x4E;  // quad_perm [2, 3, 0, 1]
 
+typedef __attribute__((address_space(3))) double* lds_dptr;
+
 // Waits for this wave's outstanding LDS operations (a cheap guard where lanes
 // of one wave exchange data through LDS outside the hot loops).
 __device__ __forceinline__ void lds_wait() { __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory"); }
@@ -165,13 +167,15 @@ struct NoBackground {
 // Cholesky of the 64 x 64 block in S (pitch LP), right-looking over four
 // 16-column panels.  Wave 0 factors a panel (all 64 rows, lane i = row i, the
 // panel's 16 entries in registers): the pivot of column c comes from lane c
-// by readlane, 1/sqrt(pivot) from v_rsq_f64 with two Newton steps (sqrt as
-// p * rsqrt with one correction).  The next column's multiplier also goes by
-// readlane, which keeps the pivot chain short; the later columns' come from
-// the column's LDS copy colv in one batch of broadcast reads.  Meanwhile
-// waves 1-3 run bg(p).  Then the four waves apply the panel to the trailing
+// by readlane, 1/sqrt(pivot) from v_rsq_f64 with one third-order correction
+// (sqrt as p * rsqrt with one correction, off the chain).  The next column's
+// multiplier also goes by readlane, which keeps the pivot chain short; the
+// later columns' come from the column's LDS copy colv in one batch of
+// broadcast reads, applied during the next column's chain so their latency is
+// hidden.  Meanwhile waves 1-3 run bg(p).  Then the four waves apply the panel to the trailing
 // 16 x 16 blocks with FP64 MFMA (rank 16 per block).  No branches on the
-// data: a non-positive or NaN pivot is recorded and replaced by 1.  Leaves L
+// data: the first non-positive or NaN pivot is recorded (what follows it is
+// meaningless and the caller discards it).  Leaves L
 // in S (upper part zeroed) and 1 / L_ii in rdiag; *fail_s / *nan_s (LDS) get
 // the first failing column (or -1) and whether its pivot was NaN.
 template <typename BG = NoBackground>
@@ -182,48 +186,66 @@ __device__ __forceinline__ void chol64_blocked(double* S, double* rdiag, double*
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     if (wv == 0) {
-      double a[16];
+      double a[16], cv[16];
 #pragma unroll
       for (int m = 0; m < 16; ++m) a[m] = S[lane * LP + 16 * p + m];
+      // colv through an opaque LDS base: the broadcast reads then share one
+      // address register and differ in their immediate offsets
+      uint32_t cb = (uint32_t)(uintptr_t)(lds_dptr)(colv + 16 * p);
+      __asm__ __volatile__("" : "+v"(cb));
+      const lds_dptr cvb = (lds_dptr)(uintptr_t)cb;
+      double lprev = 0.0, pvl = 1.0;
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
         const int cc = 16 * p + c;
-        double pv = readlane_d(a[c], cc);
-        const bool bad = !(pv > 0.0);
-        nanf = (fail < 0 && bad) ? (pv != pv) : nanf;
-        fail = (fail < 0 && bad) ? cc : fail;
-        pv = bad ? 1.0 : pv;
-        double y = __builtin_amdgcn_rsq(pv);
-        const double hp = 0.5 * pv;
-        y = fma(y, fma(-hp * y, y, 0.5), y);
-        y = fma(y, fma(-hp * y, y, 0.5), y);
-        double dg = pv * y;
-        dg = fma(fma(-dg, dg, pv), 0.5 * y, dg);
-        const double l = lane == cc ? dg : a[c] * y;
-        a[c] = lane >= cc ? l : 0.0;
-        // rows above the panel and entries right of the diagonal take garbage;
-        // only the lower part is kept
+        // pivot chain: pv -> y = 1/sqrt(pv) -> l -> next column's pivot
+        const double pv = readlane_d(a[c], cc);
+        pvl = lane == cc ? pv : pvl;  // the lane's own pivot, for L_ii and the checks
+        // v_rsq_f64 refined by one third-order step: with e = 1 - pv y0^2,
+        // 1/sqrt(pv) = y0 (1 + e/2 + 3e^2/8 + O(e^3))
+        const double y0 = __builtin_amdgcn_rsq(pv);
+        const double e = fma(-pv * y0, y0, 1.0);
+        const double y = fma(y0 * e, fma(e, 0.375, 0.5), y0);
+        const double l = a[c] * y;
         if (c + 1 < 16) {
-          // a wave's LDS accesses are performed in issue order: the reads see
-          // this column's colv and the previous column's reads are done
-          colv[lane] = l;
+          // column c - 1's broadcast (read during the chain) for column c + 1
+          if (c >= 1) a[c + 1] = fma(-lprev, cv[c + 1], a[c + 1]);
           a[c + 1] = fma(-l, readlane_d(l, cc + 1), a[c + 1]);
-          if (c + 2 < 16) {
-            double cv[16];
+          // and for the columns after it
+          if (c >= 1) {
 #pragma unroll
-            for (int m = 0; m < 16; m += 2) {
-              const v2d v = *(const v2d*)(colv + 16 * p + m);
+            for (int m = c + 2; m < 16; ++m) a[m] = fma(-lprev, cv[m], a[m]);
+          }
+          if (c + 2 < 16) {
+            // a wave's LDS accesses are performed in issue order: the reads
+            // see this column's colv and the previous column's reads are done
+            colv[lane] = l;
+            __asm__ __volatile__("" ::: "memory");
+#pragma unroll
+            for (int m = (c + 2) & ~1; m < 16; m += 2) {
+              const v2d v = *(const __attribute__((address_space(3))) v2d*)(cvb + m);
               cv[m] = v.x;
               cv[m + 1] = v.y;
             }
-#pragma unroll
-            for (int m = c + 2; m < 16; ++m) a[m] = fma(-l, cv[m], a[m]);
           }
-          __asm__ __volatile__("" ::: "memory");
         }
+        lprev = l;
+        a[c] = l;  // rows above the diagonal are garbage, masked when stored
       }
 #pragma unroll
-      for (int m = 0; m < 16; ++m) S[lane * LP + 16 * p + m] = 16 * p + m <= lane ? a[m] : 0.0;
+      for (int m = 0; m < 16; ++m) S[lane * LP + 16 * p + m] = 16 * p + m < lane ? a[m] : 0.0;
+      // L_ii = sqrt(pivot_i) = pv y, rounded by one correction; the first
+      // non-positive or NaN pivot of the panel is the failing column
+      const bool mine = lane >= 16 * p && lane < 16 * p + 16;
+      const double yl = __builtin_amdgcn_rsq(pvl);
+      double dg = pvl * fma(yl * fma(-pvl * yl, yl, 1.0), 0.5, yl);
+      dg = fma(fma(-dg, dg, pvl), 0.5 * yl, dg);
+      if (mine) S[lane * LP + lane] = dg;
+      const uint64_t badm = __ballot(mine && !(pvl > 0.0));
+      if (fail < 0 && badm) {
+        fail = __builtin_ctzll(badm);
+        nanf = __builtin_amdgcn_readlane((int)(pvl != pvl), fail);
+      }
     } else {
       bg(p);
     }
