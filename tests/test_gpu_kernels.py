@@ -93,6 +93,34 @@ def test_gemm_matches_torch(flags, shape, f32):
         torch.testing.assert_close(got, ref, rtol=1e-12, atol=1e-11 * K**0.5)
 
 
+@pytest.mark.parametrize("flags", [_lib.GEMM_B_KN | _lib.GEMM_B_UPPER,
+                                   _lib.GEMM_A_LOWER | _lib.GEMM_B_KN])
+@pytest.mark.parametrize("n,batch", [(1024, 2), (2048, 1)])
+def test_gemm_triangular_split_k(flags, n, batch):
+    """Triangular FP64 products without C0 (the shapes of the triangular
+    inversion levels), where the longest tiles are cut in two and added with
+    atomics: against torch, and bit-identical over repeated calls."""
+    _say(f"gemm split-K flags={flags} n={n} batch={batch}")
+    g = torch.Generator().manual_seed(n + flags)
+    A = torch.randn(batch, n, n, generator=g, dtype=torch.float64)
+    Bnk = torch.randn(batch, n, n, generator=g, dtype=torch.float64)
+    if flags & _lib.GEMM_A_LOWER:
+        A = torch.tril(A)
+    opB = torch.triu(Bnk) if flags & _lib.GEMM_B_UPPER else Bnk
+    ref = -1.0 * A @ opB.transpose(-1, -2)
+    Ad = A.to(DEV).contiguous()
+    Bd = opB.transpose(-1, -2).contiguous().to(DEV)  # K x N storage (B_KN)
+    outs = []
+    for _ in range(3):
+        C = torch.full((batch, n, n), 3.0, dtype=torch.float64, device=DEV)
+        _lib.check(_lib.lib().bta_gemm(flags, n, n, n, -1.0, _lib.ptr(Ad), n, _lib.ptr(Bd), n,
+                                       0.0, None, 0, _lib.ptr(C), n, batch, n * n, n * n, 0,
+                                       n * n, _lib.stream()))
+        outs.append(C.cpu())
+    torch.testing.assert_close(outs[0], ref, rtol=1e-12, atol=1e-11 * n**0.5)
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+
+
 def test_gemm_batched_lower_output():
     _say("gemm batched C_LOWER")
     L = _lib.lib()
