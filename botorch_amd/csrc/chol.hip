@@ -554,10 +554,12 @@ __global__ __launch_bounds__(256, 2) void potrf_step_kernel(StepArgs a) {
 
 // W_jj = L_jj^-1 for every 64 x 64 diagonal block of L (one workgroup per
 // block), into Linv's diagonal blocks; the rest of each block row (the upper
-// triangle of L^-1) is zeroed.
+// triangle of L^-1) is zeroed, or with band_only just the block right of an
+// even diagonal block: the upper part inside the 128-wide diagonal band, all
+// that the triangular products on 128-row tiles read.
 __global__ __launch_bounds__(256) void trinv_diag_kernel(const double* L, int64_t ldl,
                                                          double* Linv, int64_t ldinv,
-                                                         int64_t n_cols) {
+                                                         int64_t n_cols, int band_only) {
   __shared__ double S[LEAF * LP], Wl[LEAF * LP], rdiag[LEAF];
   const int64_t o = (int64_t)blockIdx.x * LEAF;
   stage64(L + o * ldl + o, ldl, S);
@@ -570,7 +572,8 @@ __global__ __launch_bounds__(256) void trinv_diag_kernel(const double* L, int64_
     Linv[(o + r) * ldinv + o + c] = Wl[r * LP + c];
   }
   // the block row right of the diagonal block is zero in L^-1
-  const int64_t rest = (int64_t)n_cols - o - LEAF;
+  int64_t rest = (int64_t)n_cols - o - LEAF;
+  if (band_only) rest = (blockIdx.x & 1) ? 0 : min(rest, (int64_t)LEAF);
   for (int r = 0; r < LEAF; ++r)
     for (int64_t c = threadIdx.x; c < rest; c += 256) Linv[(o + r) * ldinv + o + LEAF + c] = 0.0;
 }
@@ -716,7 +719,7 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
       }
       if (Linv) {
         hipLaunchKernelGGL(trinv_diag_kernel, dim3((unsigned)NB), dim3(256), 0, s, A, lda, Linv,
-                           ldinv, n_pad);
+                           ldinv, n_pad, (opts & POTRF_BAND_ZERO) ? 1 : 0);
         BTA_LAUNCH_CHECK("trinv_diag_kernel");
       }
       if (jitter_used) *jitter_used = jit;

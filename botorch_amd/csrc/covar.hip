@@ -84,6 +84,86 @@ __global__ __launch_bounds__(256) void covar_kernel(const double* __restrict__ X
   }
 }
 
+// Symmetric case for the factorisation: K = s k(Xs, Xs) + diag_add I on and
+// below the diagonal only (the Cholesky reads nothing above it), identity on
+// the padding, kernel kind fixed at compile time.  Blocks wholly above the
+// diagonal exit at once.
+template <int KIND, int MD, int ROWS>
+__global__ __launch_bounds__(256) void covar_sym_lower_kernel(const double* __restrict__ Xs,
+                                                              int64_t n, int d, double s,
+                                                              double diag_add,
+                                                              double* __restrict__ K,
+                                                              int64_t ldk, int64_t n_pad) {
+  __shared__ double xr[ROWS * MD];
+  const int64_t r0 = (int64_t)blockIdx.y * ROWS;
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if ((int64_t)blockIdx.x * 256 > r0 + ROWS - 1) return;  // block-uniform
+  for (int i = threadIdx.x; i < ROWS * MD; i += 256) {
+    const int64_t r = r0 + i / MD;
+    const int j = i % MD;
+    xr[i] = (r < n && j < d) ? Xs[r * d + j] : 0.0;
+  }
+  __syncthreads();
+  if (c >= n_pad) return;
+  double t[MD];
+  const bool cvalid = c < n;
+#pragma unroll
+  for (int j = 0; j < MD; ++j) t[j] = (j < d && cvalid) ? Xs[c * d + j] : 0.0;
+#pragma unroll 2
+  for (int rr = 0; rr < ROWS; ++rr) {
+    const int64_t r = r0 + rr;
+    if (r >= n_pad) break;
+    if (c > r) continue;
+    double v;
+    if (r < n && cvalid) {
+      double r2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < MD; ++j) {
+        const double df = xr[rr * MD + j] - t[j];  // zero past d
+        r2 = fma(df, df, r2);
+      }
+      v = kern_value_k<KIND>(r2, s);
+      if (r == c) v += diag_add;
+    } else {
+      v = r == c ? 1.0 : 0.0;
+    }
+    K[r * ldk + c] = v;
+  }
+}
+
+int covar_sym_lower(int kind, const double* Xs, int64_t n, int d, double s, double diag_add,
+                    double* K, int64_t ldk, int64_t n_pad, hipStream_t st) {
+  if (d < 1 || d > MAXD || n_pad < n || ldk < n_pad || n < 0) {
+    set_error("covar_sym_lower: bad sizes (d=%d n=%ld)", d, (long)n);
+    return BTA_ERR_INVALID;
+  }
+  if (n_pad == 0) return BTA_OK;
+  constexpr int ROWS = 16;
+  dim3 grid((unsigned)((n_pad + 255) / 256), (unsigned)((n_pad + ROWS - 1) / ROWS));
+  if (grid.y > 65535u) {
+    set_error("covar_sym_lower: too many rows");
+    return BTA_ERR_UNSUPPORTED;
+  }
+  const bool m52 = kind == BTA_KERNEL_MATERN52;
+  if (d <= 16) {
+    if (m52)
+      hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_MATERN52, 16, ROWS>), grid,
+                         dim3(256), 0, st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+    else
+      hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_RBF, 16, ROWS>), grid, dim3(256), 0,
+                         st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+  } else {
+    if (m52)
+      hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_MATERN52, MAXD, ROWS>), grid,
+                         dim3(256), 0, st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+    else
+      hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_RBF, MAXD, ROWS>), grid, dim3(256),
+                         0, st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+  }
+  BTA_LAUNCH_CHECK("covar_sym_lower_kernel");
+  return BTA_OK;
+}
+
 // dX1[r] = (sum_c G[r][c] * (-h(r,c)) (x1s_r - x2s_c) + extra[r]) / ls
 template <typename TG, int RB, int MD>
 __global__ __launch_bounds__(256) void covar_grad_kernel(

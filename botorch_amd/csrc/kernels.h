@@ -10,6 +10,10 @@ int covar(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2
           double s, double diag_add, bool f32, void* K, int64_t ldk, int64_t n1_pad,
           int64_t n2_pad, hipStream_t st, int pad_identity = 0, int batch = 1,
           int64_t sX1 = 0, int64_t sX2 = 0, int64_t sK = 0);
+// Lower triangle (diagonal included) of s k(Xs, Xs) + diag_add I, identity
+// on the padding rows/columns; nothing above the diagonal is written.
+int covar_sym_lower(int kind, const double* Xs, int64_t n, int d, double s, double diag_add,
+                    double* K, int64_t ldk, int64_t n_pad, hipStream_t st);
 int covar_grad(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2, int d,
                double s, const void* G, bool g_f32, int64_t ldg, const double* extra,
                const double* ls, double* dX1, hipStream_t st, const double* dext = nullptr,
@@ -21,8 +25,10 @@ size_t potrf_ws_bytes(int64_t n_pad);
 // returned at once; the caller rebuilds the matrix with jitter);
 // POTRF_KEEP_UPPER leaves the strict upper triangle as it was.  With Linv, the
 // 64 x 64 diagonal blocks of L^-1 are written and the rest of its upper
-// triangle zeroed.
-constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2;
+// triangle zeroed; POTRF_BAND_ZERO zeroes only the upper part inside the
+// 128-wide diagonal band (enough for trtri_lower and products on 128-row
+// tiles that treat Linv as lower triangular).
+constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2, POTRF_BAND_ZERO = 4;
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
           double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
           int opts = 0);

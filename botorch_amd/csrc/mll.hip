@@ -28,7 +28,7 @@
 namespace bta {
 
 constexpr int TR_B = 128;   // trace tile
-constexpr int TR_KC = 512;  // K chunk per task
+constexpr int TR_KC = 1024;  // K chunk per task
 
 struct TraceArgs {
   const double* X;  // L^-1 (lower), n_pad x n_pad, row stride ld
@@ -171,10 +171,13 @@ __global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(Trac
         red[e] + red[(MD + 2) + e] + red[2 * (MD + 2) + e] + red[3 * (MD + 2) + e];
 }
 
-// t = X r for the lower-triangular X (one wave per row, double2 loads)
+// t = X r for the lower-triangular X (one wave per row, double2 loads), and
+// lgd_i = log L_ii on the side
 __global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restrict__ X, int64_t ld,
                                                          const double* __restrict__ r, int64_t n,
-                                                         double* __restrict__ t) {
+                                                         double* __restrict__ t,
+                                                         const double* __restrict__ L,
+                                                         double* __restrict__ lgd) {
   const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
@@ -186,7 +189,10 @@ __global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restric
     if (k + 1 <= i) acc = fma(x.y, r[k + 1], acc);
   }
   acc = wave_sum(acc);
-  if (lane == 0) t[i] = acc;
+  if (lane == 0) {
+    t[i] = acc;
+    lgd[i] = log(L[i * ld + i]);
+  }
 }
 
 // part[rb][j] = sum over the 64 rows i of block rb (i >= j) of X_ij t_i
@@ -217,25 +223,46 @@ __global__ void trmv_lower_t_sum_kernel(const double* __restrict__ part, int64_t
                                         double* __restrict__ a) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_pad) return;
+  // sixteen partials loaded together per round, added in row-block order
   double s = 0.0;
-  for (int64_t rb = j / 64; rb < n_pad / 64; ++rb) s += part[rb * n_pad + j];
+  const int64_t nb = n_pad / 64;
+  for (int64_t rb0 = j / 64; rb0 < nb; rb0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = rb0 + u < nb ? part[(rb0 + u) * n_pad + j] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += v[u];
+  }
   a[j] = s;
 }
 
 // out[0] = sum t_i^2, out[1] = sum_{i<n} log L_ii, out[2] = sum_{i<n} a_i,
 // out[3 + e] = sum over the W workers of partial[w][e], e < np2.
-__global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const double* L,
-                                                         int64_t ld, const double* alpha,
-                                                         int64_t n, const double* partial,
-                                                         int W, int np2, double* out) {
+template <int NP2>
+__global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const double* lgd,
+                                                         const double* alpha, int64_t n,
+                                                         const double* partial, int W,
+                                                         double* out) {
   __shared__ double red[8 * 40];
   __shared__ double red3[4 * 3];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double q = 0.0, lg = 0.0, sa = 0.0;
-  for (int64_t i = tid; i < n; i += 256) {
-    q += t[i] * t[i];
-    lg += log(L[i * ld + i]);
-    sa += alpha[i];
+  // eight elements per thread loaded together per round
+  for (int64_t i0 = tid; i0 < n; i0 += 256 * 8) {
+    double tv[8], lv[8], av[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t i = i0 + 256 * u;
+      tv[u] = i < n ? t[i] : 0.0;
+      lv[u] = i < n ? lgd[i] : 0.0;
+      av[u] = i < n ? alpha[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      q = fma(tv[u], tv[u], q);
+      lg += lv[u];
+      sa += av[u];
+    }
   }
   q = wave_sum(q);
   lg = wave_sum(lg);
@@ -245,20 +272,25 @@ __global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const 
     red3[wv * 3 + 1] = lg;
     red3[wv * 3 + 2] = sa;
   }
-  // worker sums: group gi of 32 threads takes workers w = gi (mod 8)
-  const int gi = tid >> 5, e0 = tid & 31;
-  for (int e = e0; e < np2; e += 32) {
-    double acc = 0.0;
-    for (int w = gi; w < W; w += 8) acc += partial[(int64_t)w * np2 + e];
-    red[gi * 40 + e] = acc;
+  // worker sums: thread tid takes workers w = tid (mod 256), all of its loads
+  // in flight together, then a fixed-order reduction over lanes and waves
+  static_assert(NP2 <= 40, "red holds 40 values per wave");
+  double acc[NP2];
+#pragma unroll
+  for (int e = 0; e < NP2; ++e) acc[e] = 0.0;
+  for (int w = tid; w < W; w += 256) {
+#pragma unroll
+    for (int e = 0; e < NP2; ++e) acc[e] += partial[(int64_t)w * NP2 + e];
+  }
+#pragma unroll
+  for (int e = 0; e < NP2; ++e) {
+    const double v = wave_sum(acc[e]);
+    if (lane == 0) red[wv * 40 + e] = v;
   }
   __syncthreads();
   if (tid < 3) out[tid] = red3[tid] + red3[3 + tid] + red3[6 + tid] + red3[9 + tid];
-  for (int e = tid; e < np2; e += 256) {
-    double acc = 0.0;
-    for (int g = 0; g < 8; ++g) acc += red[g * 40 + e];
-    out[3 + e] = acc;
-  }
+  for (int e = tid; e < NP2; e += 256)
+    out[3 + e] = red[e] + red[40 + e] + red[80 + e] + red[120 + e];
 }
 
 __global__ void mll_residual_kernel(const double* y, int64_t n, double c, double* r, int64_t n_pad) {
@@ -269,7 +301,7 @@ __global__ void mll_residual_kernel(const double* y, int64_t n, double c, double
 struct MllWs {
   int64_t n_pad;
   int W, MD;
-  size_t ls, center, xs, L, Linv, fact, r, t, alpha, part, partial, out, total;
+  size_t ls, center, xs, L, Linv, fact, r, t, lgd, alpha, part, partial, out, total;
   MllWs(int64_t n, int d) {
     n_pad = round_up(n, 128);
     MD = d <= 16 ? 16 : 32;
@@ -283,6 +315,7 @@ struct MllWs {
     fact = p.take(std::max(potrf_ws_bytes(n_pad), trtri_ws_bytes(n_pad)));
     r = p.take((size_t)n_pad * 8);
     t = p.take((size_t)n_pad * 8);
+    lgd = p.take((size_t)n_pad * 8);
     alpha = p.take((size_t)n_pad * 8);
     part = p.take((size_t)(n_pad / 64) * n_pad * 8);
     partial = p.take((size_t)W * (MD + 2) * 8);
@@ -322,6 +355,7 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   const size_t fact_bytes = std::max(potrf_ws_bytes(w.n_pad), trtri_ws_bytes(w.n_pad));
   double* r = (double*)(base + w.r);
   double* t = (double*)(base + w.t);
+  double* lgd = (double*)(base + w.lgd);
   double* alpha = (double*)(base + w.alpha);
   double* part = (double*)(base + w.part);
   double* partial = (double*)(base + w.partial);
@@ -336,12 +370,11 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   const double jitter0 = 1e-8;
   for (int attempt = 0;; ++attempt) {
     const double jit = attempt == 0 ? 0.0 : jitter0 * std::pow(10.0, (double)(attempt - 1));
-    rc = covar(kind, xs, n, xs, n, d, s, noise + jit, false, L, np, np, np, st,
-               /*pad_identity=*/1);
+    rc = covar_sym_lower(kind, xs, n, d, s, noise + jit, L, np, np, st);
     if (rc) return rc;
     double used = 0.0;
     rc = potrf(L, np, np, Linv, np, 0, jitter0, &used, fact, fact_bytes, st,
-               POTRF_NO_SAVE | POTRF_KEEP_UPPER);
+               POTRF_NO_SAVE | POTRF_KEEP_UPPER | POTRF_BAND_ZERO);
     if (rc != BTA_ERR_NOT_PSD) break;
     if (attempt >= max_tries) {
       set_error("mll_value_grad: matrix not positive definite after adding jitter up to %.1e",
@@ -356,7 +389,7 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
                      c, r, np);
   BTA_LAUNCH_CHECK("mll_residual_kernel");
   hipLaunchKernelGGL(trmv_lower_kernel, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, Linv,
-                     np, r, np, t);
+                     np, r, np, t, L, lgd);
   BTA_LAUNCH_CHECK("trmv_lower_kernel");
   {
     const unsigned gx = (unsigned)std::min<int64_t>((np + 255) / 256, 16);
@@ -386,8 +419,12 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   else
     launch_trace<32>(kind, ta, W, st);
   BTA_LAUNCH_CHECK("mll_trace_kernel");
-  hipLaunchKernelGGL(mll_finish_kernel, dim3(1), dim3(256), 0, st, t, L, np, alpha, n, partial, W,
-                     w.MD + 2, out);
+  if (w.MD == 16)
+    hipLaunchKernelGGL(mll_finish_kernel<18>, dim3(1), dim3(256), 0, st, t, lgd, alpha, n, partial,
+                       W, out);
+  else
+    hipLaunchKernelGGL(mll_finish_kernel<34>, dim3(1), dim3(256), 0, st, t, lgd, alpha, n, partial,
+                       W, out);
   BTA_LAUNCH_CHECK("mll_finish_kernel");
   std::vector<double> h(3 + w.MD + 2);
   BTA_HIP(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
