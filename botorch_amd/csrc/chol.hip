@@ -378,6 +378,8 @@ struct StepArgs {
   double* A;
   int64_t lda;
   int* info;
+  double* Linv;  // when set: the diagonal workgroup also writes L_jj^-1 into
+  int64_t ldinv;  // Linv's diagonal block and zeroes the block right of it
   int j, NB, nstrip;
 };
 
@@ -469,6 +471,18 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
       const int r = e >> 6, c = e & 63;
       if (c <= r) Ajj[(int64_t)r * lda + c] = S[r * LP + c];
     }
+    if (a.Linv) {
+      // the diagonal workgroup finishes before the others: invert here (Bf,
+      // the staged L_j,j-1, is no longer needed)
+      trinv64_blocked(S, rdiag, Bf);
+      double* W = a.Linv + (int64_t)j * LEAF * a.ldinv + (int64_t)j * LEAF;
+      const bool zr = j + 1 < a.NB;
+      for (int e = tid; e < LEAF * LEAF; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        W[(int64_t)r * a.ldinv + c] = Bf[r * LP + c];
+        if (zr) W[(int64_t)r * a.ldinv + LEAF + c] = 0.0;
+      }
+    }
     return;
   }
 #pragma unroll
@@ -554,12 +568,10 @@ __global__ __launch_bounds__(256, 2) void potrf_step_kernel(StepArgs a) {
 
 // W_jj = L_jj^-1 for every 64 x 64 diagonal block of L (one workgroup per
 // block), into Linv's diagonal blocks; the rest of each block row (the upper
-// triangle of L^-1) is zeroed, or with band_only just the block right of an
-// even diagonal block: the upper part inside the 128-wide diagonal band, all
-// that the triangular products on 128-row tiles read.
+// triangle of L^-1) is zeroed.
 __global__ __launch_bounds__(256) void trinv_diag_kernel(const double* L, int64_t ldl,
                                                          double* Linv, int64_t ldinv,
-                                                         int64_t n_cols, int band_only) {
+                                                         int64_t n_cols) {
   __shared__ double S[LEAF * LP], Wl[LEAF * LP], rdiag[LEAF];
   const int64_t o = (int64_t)blockIdx.x * LEAF;
   stage64(L + o * ldl + o, ldl, S);
@@ -572,8 +584,7 @@ __global__ __launch_bounds__(256) void trinv_diag_kernel(const double* L, int64_
     Linv[(o + r) * ldinv + o + c] = Wl[r * LP + c];
   }
   // the block row right of the diagonal block is zero in L^-1
-  int64_t rest = (int64_t)n_cols - o - LEAF;
-  if (band_only) rest = (blockIdx.x & 1) ? 0 : min(rest, (int64_t)LEAF);
+  const int64_t rest = (int64_t)n_cols - o - LEAF;
   for (int r = 0; r < LEAF; ++r)
     for (int64_t c = threadIdx.x; c < rest; c += 256) Linv[(o + r) * ldinv + o + LEAF + c] = 0.0;
 }
@@ -645,6 +656,35 @@ size_t potrf_ws_bytes(int64_t n_pad) {
   return (size_t)(n_pad * n_pad + LEAF * LEAF) * sizeof(double) + 256;
 }
 
+// Step j of the factorisation of the NB x NB block matrix A on a stream whose
+// launches reach ncu CUs.
+static int launch_step(double* A, int64_t lda, int* info, double* Linv, int64_t ldinv, int j,
+                       int NB, int ncu, hipStream_t s) {
+  StepArgs sa;
+  sa.A = A;
+  sa.lda = lda;
+  sa.info = info;
+  sa.Linv = Linv;
+  sa.ldinv = ldinv;
+  sa.j = j;
+  sa.NB = NB;
+  sa.nstrip = NB - j;
+  int bulk = 0;
+  if (j > 0 && NB - j - 1 > 0) {
+    // The strip workgroups (dispatched first, one per CU) are the step's
+    // critical path once the update is small: if at most two rounds of tiles
+    // fit on the CUs they leave free, keep every strip workgroup alone on its
+    // CU; otherwise fill the second slot of every CU.
+    const int MT = (NB - j - 1 + 1) / 2;  // 128-row tile rows over the trailing blocks
+    const int tiles = MT * (MT + 1);
+    const int w1 = std::max(ncu - sa.nstrip, 1), w2 = std::max(2 * ncu - sa.nstrip, 1);
+    bulk = std::min(tiles, (tiles + w1 - 1) / w1 <= 2 ? w1 : w2);
+  }
+  hipLaunchKernelGGL(potrf_step_kernel, dim3((unsigned)(sa.nstrip + bulk)), dim3(256), 0, s, sa);
+  BTA_LAUNCH_CHECK("potrf_step_kernel");
+  return BTA_OK;
+}
+
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
           double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
           int opts) {
@@ -686,27 +726,9 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, info, 2, 0);
     BTA_LAUNCH_CHECK("fill_int_kernel");
     for (int j = 0; j < NB; ++j) {
-      StepArgs sa;
-      sa.A = A;
-      sa.lda = lda;
-      sa.info = info;
-      sa.j = j;
-      sa.NB = NB;
-      sa.nstrip = NB - j;
-      int bulk = 0;
-      if (j > 0 && NB - j - 1 > 0) {
-        // The strip workgroups (dispatched first, one per CU) are the step's
-        // critical path once the update is small: if at most two rounds of
-        // tiles fit on the CUs they leave free, keep every strip workgroup
-        // alone on its CU; otherwise fill the second slot of every CU.
-        const int MT = (NB - j - 1 + 1) / 2;  // 128-row tile rows over the trailing blocks
-        const int tiles = MT * (MT + 1), ncu = num_cus();
-        const int w1 = std::max(ncu - sa.nstrip, 1), w2 = std::max(2 * ncu - sa.nstrip, 1);
-        bulk = std::min(tiles, (tiles + w1 - 1) / w1 <= 2 ? w1 : w2);
-      }
-      hipLaunchKernelGGL(potrf_step_kernel, dim3((unsigned)(sa.nstrip + bulk)), dim3(256), 0, s,
-                         sa);
-      BTA_LAUNCH_CHECK("potrf_step_kernel");
+      const int rc = launch_step(A, lda, info, (opts & POTRF_BAND_ZERO) ? Linv : nullptr, ldinv, j,
+                                 NB, num_cus(), s);
+      if (rc) return rc;
     }
     int hinfo[2];
     BTA_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s));
@@ -717,9 +739,9 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
         hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, A, lda, n_pad);
         BTA_LAUNCH_CHECK("zero_upper_kernel");
       }
-      if (Linv) {
+      if (Linv && !(opts & POTRF_BAND_ZERO)) {  // (with it, the steps wrote them)
         hipLaunchKernelGGL(trinv_diag_kernel, dim3((unsigned)NB), dim3(256), 0, s, A, lda, Linv,
-                           ldinv, n_pad, (opts & POTRF_BAND_ZERO) ? 1 : 0);
+                           ldinv, n_pad);
         BTA_LAUNCH_CHECK("trinv_diag_kernel");
       }
       if (jitter_used) *jitter_used = jit;

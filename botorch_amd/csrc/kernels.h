@@ -25,9 +25,10 @@ size_t potrf_ws_bytes(int64_t n_pad);
 // returned at once; the caller rebuilds the matrix with jitter);
 // POTRF_KEEP_UPPER leaves the strict upper triangle as it was.  With Linv, the
 // 64 x 64 diagonal blocks of L^-1 are written and the rest of its upper
-// triangle zeroed; POTRF_BAND_ZERO zeroes only the upper part inside the
-// 128-wide diagonal band (enough for trtri_lower and products on 128-row
-// tiles that treat Linv as lower triangular).
+// triangle zeroed; POTRF_BAND_ZERO zeroes only the 64 x 64 blocks right of
+// the diagonal blocks (enough for trtri_lower and for products on 128-row
+// tiles, at any block offset, that treat Linv as lower triangular), and the
+// diagonal blocks are inverted inside the factorisation steps.
 constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2, POTRF_BAND_ZERO = 4;
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
           double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
