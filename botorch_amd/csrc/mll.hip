@@ -124,11 +124,12 @@ __global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(Trac
           const bool ok = jok && gi < a.n && gi >= gj;
           const double wij = (first ? a.alpha[min(gi, a.n_pad - 1)] * aj : 0.0) - pij;
           const double f = ok ? (gi == gj ? 1.0 : 2.0) : 0.0;
-          double r2 = 0.0;
+          double r2 = 0.0, dsq[MD];
 #pragma unroll
           for (int j = 0; j < MD; ++j) {
             const double df = xr[rl * MD + j] - xj[j];
-            r2 = fma(df, df, r2);
+            dsq[j] = df * df;
+            r2 += dsq[j];
           }
           double kv, h;
           if (KIND == BTA_KERNEL_MATERN52) {
@@ -146,10 +147,7 @@ __global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(Trac
           acc_noise += (ok && gi == gj) ? wij : 0.0;
           const double fwh = fw * h;
 #pragma unroll
-          for (int j = 0; j < MD; ++j) {
-            const double df = xr[rl * MD + j] - xj[j];
-            acc_ls[j] = fma(fwh, df * df, acc_ls[j]);
-          }
+          for (int j = 0; j < MD; ++j) acc_ls[j] = fma(fwh, dsq[j], acc_ls[j]);
         }
     }
     __syncthreads();
