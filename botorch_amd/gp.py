@@ -107,6 +107,14 @@ class _HyperModule(torch.nn.Module):
         raw = getattr(self, f"raw_{name}")
         c = getattr(self, f"raw_{name}_constraint")
         v = torch.as_tensor(value, dtype=raw.dtype, device=raw.device).expand_as(raw)
+        if isinstance(c, GreaterThan) and c.transform is not None and bool(
+                (v < c.lower_bound).any()):
+            # gpytorch's Module.initialize refuses values outside a constraint
+            raise RuntimeError(
+                f"Attempting to manually set a parameter value that is out of bounds of its "
+                f"current constraints, {type(c).__name__}({c.lower_bound:.3E}). Most likely, "
+                f"you want to do the following:\n likelihood = GaussianLikelihood"
+                f"(noise_constraint=gpytorch.constraints.GreaterThan(better_lower_bound))")
         with torch.no_grad():
             raw.copy_(c.inverse_transform(v))
 

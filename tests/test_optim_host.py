@@ -52,3 +52,14 @@ def test_initialize_q_batch_keeps_best():
     Xs, ys = initialize_q_batch(X, y, 10, generator=torch.Generator().manual_seed(1))
     assert Xs.shape == (10, 2, 3)
     assert y.max() in ys
+
+
+def test_constraint_violating_hyperparameter_raises():
+    """As gpytorch's Module.initialize: a value below a GreaterThan bound is
+    refused instead of producing a NaN raw parameter."""
+    import botorch_amd as B
+    lik = B.GaussianLikelihood()
+    with pytest.raises(RuntimeError, match="out of bounds"):
+        lik.noise = 1e-6
+    lik.noise = 1e-4  # the bound itself is allowed
+    assert abs(float(lik.noise.detach()) - 1e-4) < 1e-12
