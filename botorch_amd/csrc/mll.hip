@@ -28,7 +28,7 @@
 namespace bta {
 
 constexpr int TR_B = 128;   // trace tile
-constexpr int TR_KC = 1024;  // K chunk per task
+constexpr int TR_KC = 4096;  // K chunk per task
 
 struct TraceArgs {
   const double* X;  // L^-1 (lower), n_pad x n_pad, row stride ld

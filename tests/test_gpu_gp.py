@@ -249,3 +249,43 @@ def test_mll_full_size_n4096_d16():
                          lsr.grad.cpu()])
         assert abs(got[0] - ref[0]) <= 1e-10 * abs(ref[0])
         assert _rel(got[1:], ref[1:]) < 1e-8
+
+
+def test_mll_gradient_matches_central_fd():
+    """The HIP MLL gradient with respect to log-hyperparameters against central
+    differences of the HIP log-likelihood (h = 1e-6 on the log scale)."""
+    _say("mll central FD n=512 d=6")
+    import ctypes
+    from botorch_amd import _lib
+    g = torch.Generator().manual_seed(9)
+    n, d = 512, 6
+    X = torch.rand(n, d, generator=g, dtype=torch.float64).to(DEV)
+    y = (torch.sin(3 * X).sum(-1) + 0.05 * torch.randn(n, generator=g, dtype=torch.float64)
+         .to(DEV))
+    y = ((y - y.mean()) / y.std()).contiguous()
+    L = _lib.lib()
+    ws = _lib.workspace(L.bta_mll_workspace_size(n, d), DEV)
+
+    def run(theta):
+        ls, os_, noise = theta[:d], theta[d], theta[d + 1]
+        out = (ctypes.c_double * (4 + d))()
+        lsh = (ctypes.c_double * d)(*ls)
+        _lib.check(L.bta_mll_value_grad(O.MATERN52, _lib.ptr(X), _lib.ptr(y), n, d, lsh, os_,
+                                        noise, 0.0, 6, out, _lib.ptr(ws), ws.numel(),
+                                        _lib.stream()))
+        return list(out)
+
+    theta = [0.3 + 0.05 * i for i in range(d)] + [1.3, 0.02]
+    out = run(theta)
+    # analytic d/dlog(theta) = theta * d/dtheta; out = [ll, d/ds, d/dnoise, d/dc, d/dls...]
+    analytic = [theta[j] * out[4 + j] for j in range(d)] + [theta[d] * out[1],
+                                                           theta[d + 1] * out[2]]
+    h = 1e-6
+    fd = []
+    for j in range(d + 2):
+        tp, tm = list(theta), list(theta)
+        tp[j] *= math.exp(h)
+        tm[j] *= math.exp(-h)
+        fd.append((run(tp)[0] - run(tm)[0]) / (2 * h))
+    a, f = torch.tensor(analytic), torch.tensor(fd)
+    assert _rel(a, f) < 1e-6, (a, f)
