@@ -7,8 +7,9 @@
 // K^-1 = X^T X with X = L^-1 is never stored: mll_trace_kernel forms each
 // lower 128 x 64 tile of it on the FP64 MFMA main loop (d_mainloop, both
 // operands read K-major straight from X), the K range of a tile,
-// [max(128 tm, 64 tn), n_pad), cut into chunks of KC so that every task costs
-// about the same.  Each task's epilogue rebuilds k and dk/dtheta for the tile's point
+// [max(128 tm, 64 tn), n_pad), cut into chunks of KC so that no task grows
+// without bound at large n; the workers pair long and short tasks on each
+// CU.  Each task's epilogue rebuilds k and dk/dtheta for the tile's point
 // pairs from the scaled inputs staged in LDS and folds -P_ij dK_ij/dtheta
 // (plus a_i a_j dK_ij/dtheta for the tile's first chunk; the sum is linear in
 // K^-1) into per-worker sums, so the n^2 x (d + 2) derivative tensor is never
