@@ -239,6 +239,16 @@ def main():
     v_m, g_m, st_m = acq_m._launch(X, True)
     acq_m.check_status(st_m)
     log(f"mixed: {t_mixed * 1e3:.3f} ms/step")
+    # the same step captured once in a HIP graph and replayed (inputs copied
+    # into the captured buffer by the caller between steps)
+    Xg = X.clone()
+    acq_m._launch(Xg, True)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        acq_m._launch(Xg, True)
+    t_graph = _timed(graph.replay, args.steps, args.warmup)
+    log(f"mixed, graph replay: {t_graph * 1e3:.3f} ms/step")
     # ---------------- FP64 and FP32-contraction HIP paths -------------------------
     acq_d, *_ = build_headline("fp64", n, d, b, q, S)
     t_fp64 = _timed(lambda: acq_d._launch(X, True), max(10, args.steps // 2), args.warmup)
@@ -345,6 +355,7 @@ def main():
                    "n": n, "d": d, "restarts": b, "q": q, "mc_samples": S, "n_baseline": n_b,
                    "precision": "mixed"},
         "restart_evals_per_s": b / t_mixed,
+        "mixed_graph_replay_ms_per_step": t_graph * 1e3,
         "fp64_hip_steps_per_s": 1.0 / t_fp64,
         "mixed_f32_hip_steps_per_s": 1.0 / t_f32,
         "roofline_mixed_f32_gemm": roof_f32,

@@ -162,3 +162,28 @@ def test_qlognei_mixed_modes_headline_shape(precision):
     eg = ((gr.cpu() - g_ref).abs().max() / g_ref.abs().max()).item()
     print(f"{precision}: value {ev:.2e} grad {eg:.2e}")
     assert ev < 1e-4 and eg < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_qlognei_step_graph_replay(precision):
+    """One value+grad step captured in a HIP graph (torch.cuda.CUDAGraph) and
+    replayed with new inputs copied into the captured input buffer: every
+    replay equals the eager step bit for bit (eager work and synchronisation
+    between the replays)."""
+    _say(f"qlognei graph replay n=1024 d=8 q=2 b=64 {precision}")
+    acq, _, Xq = _acq_pair(1024, 8, 2, 256, precision, seed=5)
+    Xs = Xq[:64].to(DEV).contiguous()
+    acq._launch(Xs, True)  # workspace, base samples and GEMM plans exist before capture
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        v_g, g_g, st_g = acq._launch(Xs, True)
+    g = torch.Generator().manual_seed(11)
+    for _ in range(3):
+        Xn = torch.rand(Xs.shape, generator=g, dtype=torch.float64).to(DEV)
+        Xs.copy_(Xn)
+        graph.replay()
+        torch.cuda.synchronize()
+        v_e, g_e, st_e = acq._launch(Xn.clone(), True)
+        torch.cuda.synchronize()
+        assert torch.equal(v_g, v_e) and torch.equal(g_g, g_e) and torch.equal(st_g, st_e)
