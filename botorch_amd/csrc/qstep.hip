@@ -86,6 +86,26 @@ __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
   const int64_t c0 = (int64_t)ct * QS_CT, r0 = (int64_t)blockIdx.y * QS_RT;
   const int64_t r0w = r0 + wv * 16;
   const int64_t rl = r0w + l16;  // this lane's row as B operand / D column
+  // Every global load of the block is issued before the staging barrier: the
+  // raw row coordinates and the first 16 columns of E (all of them for
+  // ne <= 16), so that their latency overlaps the staging of T.
+  double xr[KD], xc[KD], xl[KD];
+#pragma unroll
+  for (int s = 0; s < KD; ++s) {
+    const int j = 4 * s + g;
+    const bool ok = j < d && rl < a.R;
+    xr[s] = ok ? a.X[rl * d + j] : 0.0;
+    xc[s] = ok ? a.center[j] : 0.0;
+    xl[s] = ok ? a.ls[j] : 1.0;
+  }
+  double ev[4][4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t c = c0 + cb * 16 + g + 4 * q;
+      ev[cb][q] = (l16 < a.ne && c < a.n) ? a.E[c * a.ne + l16] : 0.0;
+    }
   qs_stage_t<KD>(sm, a.T, a.n, d, c0);
 
   // scaled row coordinates: lane (g, l16) holds x[rl][4s + g]
@@ -94,8 +114,7 @@ __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
 #pragma unroll
   for (int s = 0; s < KD; ++s) {
     const int j = 4 * s + g;
-    double v = 0.0;
-    if (j < d && rl < a.R) v = (a.X[rl * d + j] - a.center[j]) / a.ls[j];
+    const double v = (xr[s] - xc[s]) / xl[s];
     xb[s] = v;
     xn2 = fma(v, v, xn2);
     if (ct == 0 && j < d) a.xs_out[rl * d + j] = v;
@@ -148,7 +167,8 @@ __global__ __launch_bounds__(256) void qstep_covar_kernel(QsCovarArgs a) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int64_t c = c0 + cb * 16 + g + 4 * q;
-        const double e = (jj < a.ne && c < a.n) ? a.E[c * a.ne + jj] : 0.0;
+        const double e =
+            j0 == 0 ? ev[cb][q] : ((jj < a.ne && c < a.n) ? a.E[c * a.ne + jj] : 0.0);
         p = mfma64(kv[cb][q], e, p);
       }
     // D layout: lane (g, l16) reg q = P[r0w + g + 4q][jj]
