@@ -127,6 +127,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// NV independent wave sums with the shuffles of each butterfly step issued
+// together: one cross-lane latency per step instead of one per value and step.
+// Each value is summed in the same order as wave_sum.
+template <int NV>
+__device__ __forceinline__ void wave_sum_n(double (&v)[NV]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double t[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) t[i] = __shfl_xor(v[i], off, 64);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] += t[i];
+  }
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));

@@ -155,15 +155,15 @@ __global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(Trac
   }
   // per-worker reduction
   double* red = (double*)lds;
-  double v = wave_sum(acc_os);
-  if (lane == 0) red[wv * (MD + 2)] = v;
-  v = wave_sum(acc_noise);
-  if (lane == 0) red[wv * (MD + 2) + 1] = v;
+  double v[MD + 2];
+  v[0] = acc_os;
+  v[1] = acc_noise;
 #pragma unroll
-  for (int j = 0; j < MD; ++j) {
-    v = wave_sum(acc_ls[j]);
-    if (lane == 0) red[wv * (MD + 2) + 2 + j] = v;
-  }
+  for (int j = 0; j < MD; ++j) v[2 + j] = acc_ls[j];
+  wave_sum_n<MD + 2>(v);
+  if (lane == 0)
+#pragma unroll
+    for (int e = 0; e < MD + 2; ++e) red[wv * (MD + 2) + e] = v[e];
   __syncthreads();
   for (int e = threadIdx.x; e < MD + 2; e += 256)
     a.partial[(int64_t)w * (MD + 2) + e] =
@@ -281,11 +281,10 @@ __global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const 
 #pragma unroll
     for (int e = 0; e < NP2; ++e) acc[e] += partial[(int64_t)w * NP2 + e];
   }
+  wave_sum_n<NP2>(acc);
+  if (lane == 0)
 #pragma unroll
-  for (int e = 0; e < NP2; ++e) {
-    const double v = wave_sum(acc[e]);
-    if (lane == 0) red[wv * 40 + e] = v;
-  }
+    for (int e = 0; e < NP2; ++e) red[wv * 40 + e] = acc[e];
   __syncthreads();
   if (tid < 3) out[tid] = red3[tid] + red3[3 + tid] + red3[6 + tid] + red3[9 + tid];
   for (int e = tid; e < NP2; e += 256)
