@@ -142,6 +142,41 @@ __device__ __forceinline__ void wave_sum_n(double (&v)[NV]) {
   }
 }
 
+// NV <= 32 wave sums with P - 1 + (6 - log2 P) shuffles instead of 6 NV
+// (P = NV rounded up to a power of two): the first log2 P butterfly steps
+// halve the values each lane carries (the lane with the step's bit set keeps
+// the upper half, its partner the lower), the remaining steps are ordinary
+// butterflies.  The total of value e ends in lanes e * 64/P .. (e + 1) * 64/P - 1;
+// the first of them gets *e_out = e, every other lane -1 (also lanes past NV).
+// Every value is summed over the same tree as wave_sum: bit-identical totals.
+template <int NV>
+__device__ __forceinline__ double wave_sum_spread(const double (&v)[NV], int* e_out) {
+  static_assert(NV >= 1 && NV <= 32, "wave_sum_spread: 1..32 values");
+  constexpr int P = NV <= 1 ? 1 : NV <= 2 ? 2 : NV <= 4 ? 4 : NV <= 8 ? 8 : NV <= 16 ? 16 : 32;
+  constexpr int LG = P == 1 ? 0 : P == 2 ? 1 : P == 4 ? 2 : P == 8 ? 3 : P == 16 ? 4 : 5;
+  const int lane = threadIdx.x & 63;
+  double a[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i) a[i] = i < NV ? v[i] : 0.0;
+#pragma unroll
+  for (int s = 0; s < LG; ++s) {
+    const int h = P >> (s + 1), off = 32 >> s;
+    const bool up = (lane & off) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const double keep = up ? a[h + i] : a[i];
+      const double send = up ? a[i] : a[h + i];
+      a[i] = keep + __shfl_xor(send, off, 64);
+    }
+  }
+  double t = a[0];
+#pragma unroll
+  for (int off = 32 >> LG; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  const int e = lane >> (6 - LG);
+  *e_out = ((lane & ((64 >> LG) - 1)) == 0 && e < NV) ? e : -1;
+  return t;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));

@@ -160,10 +160,17 @@ __global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(Trac
   v[1] = acc_noise;
 #pragma unroll
   for (int j = 0; j < MD; ++j) v[2 + j] = acc_ls[j];
-  wave_sum_n<MD + 2>(v);
-  if (lane == 0)
+  if (MD + 2 <= 32) {
+    int e;
+    const double t = wave_sum_spread<(MD + 2 <= 32 ? MD + 2 : 1)>(
+        *(const double(*)[MD + 2 <= 32 ? MD + 2 : 1])v, &e);
+    if (e >= 0) red[wv * (MD + 2) + e] = t;
+  } else {
+    wave_sum_n<MD + 2>(v);
+    if (lane == 0)
 #pragma unroll
-    for (int e = 0; e < MD + 2; ++e) red[wv * (MD + 2) + e] = v[e];
+      for (int e = 0; e < MD + 2; ++e) red[wv * (MD + 2) + e] = v[e];
+  }
   __syncthreads();
   for (int e = threadIdx.x; e < MD + 2; e += 256)
     a.partial[(int64_t)w * (MD + 2) + e] =
@@ -281,10 +288,17 @@ __global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const 
 #pragma unroll
     for (int e = 0; e < NP2; ++e) acc[e] += partial[(int64_t)w * NP2 + e];
   }
-  wave_sum_n<NP2>(acc);
-  if (lane == 0)
+  if (NP2 <= 32) {
+    int e;
+    const double v = wave_sum_spread<(NP2 <= 32 ? NP2 : 1)>(
+        *(const double(*)[NP2 <= 32 ? NP2 : 1])acc, &e);
+    if (e >= 0) red[wv * 40 + e] = v;
+  } else {
+    wave_sum_n<NP2>(acc);
+    if (lane == 0)
 #pragma unroll
-    for (int e = 0; e < NP2; ++e) red[wv * 40 + e] = acc[e];
+      for (int e = 0; e < NP2; ++e) red[wv * 40 + e] = acc[e];
+  }
   __syncthreads();
   if (tid < 3) out[tid] = red3[tid] + red3[3 + tid] + red3[6 + tid] + red3[9 + tid];
   for (int e = tid; e < NP2; e += 256)

@@ -114,13 +114,20 @@ __device__ __forceinline__ void log_fatplus_grad(double x, double tau, double* v
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red, double (&out)[NV]) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  double s[NV];
+  if (NV <= 32) {
+    int e;
+    const double s =
+        wave_sum_spread<(NV <= 32 ? NV : 1)>(*(const double(*)[NV <= 32 ? NV : 1])v, &e);
+    if (e >= 0) red[wv * 64 + e] = s;
+  } else {
+    double s[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) s[i] = v[i];
-  wave_sum_n<NV>(s);
-  if (lane == 0)
+    for (int i = 0; i < NV; ++i) s[i] = v[i];
+    wave_sum_n<NV>(s);
+    if (lane == 0)
 #pragma unroll
-    for (int i = 0; i < NV; ++i) red[wv * 64 + i] = s[i];
+      for (int i = 0; i < NV; ++i) red[wv * 64 + i] = s[i];
+  }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NV; ++i) out[i] = red[i] + red[64 + i] + red[128 + i] + red[192 + i];
@@ -418,10 +425,11 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
         }
       }
     }
-    wave_sum_n<NV>(acc);
-    if (lane == 0)
-#pragma unroll
-      for (int e = 0; e < NV; ++e) red[wv * 64 + e] = acc[e];
+    {
+      int e;
+      const double sw = wave_sum_spread<NV>(acc, &e);
+      if (e >= 0) red[wv * 64 + e] = sw;
+    }
     __syncthreads();
     for (int e = tid; e < Q * (m + 1); e += 256) {
       const int i = e / (m + 1), k = e % (m + 1);
