@@ -176,12 +176,12 @@ __device__ bool small_chol(const double* A, double jit, double* L) {
     double s = A[j * Q + j] + jit;
     for (int k = 0; k < j; ++k) s -= L[j * Q + k] * L[j * Q + k];
     if (!(s > 0.0)) return false;
-    const double dj = sqrt(s);
+    const double dj = sqrt(s), rdj = 1.0 / dj;
     L[j * Q + j] = dj;
     for (int i = j + 1; i < Q; ++i) {
       double t = A[i * Q + j];
       for (int k = 0; k < j; ++k) t -= L[i * Q + k] * L[j * Q + k];
-      L[i * Q + j] = t / dj;
+      L[i * Q + j] = t * rdj;
     }
   }
   return true;
@@ -221,13 +221,22 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   double* pv = sm + lay.pv;  // Q x (n_b + 1): P[r0 + i][j]
   double* vv = sm + lay.vv;  // packed lower triangle of V_b V_b^T
   {
+    // four tasks per wave and round, their loads issued together
     const int ne = n_b + 1;
-    for (int t = wv; t < Q * ne; t += 4) {
-      double s = 0.0;
-      const double* p = a.P + (r0 + t / ne) * ne + t % ne;
-      for (int pt = lane; pt < a.p_parts; pt += 64) s += p[pt * a.p_stride];
-      s = wave_sum(s);
-      if (lane == 0) pv[t] = s;
+    for (int t0 = wv * 4; t0 < Q * ne; t0 += 16) {
+      double s[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u;
+        s[u] = 0.0;
+        if (t < Q * ne) {
+          const double* p = a.P + (r0 + t / ne) * ne + t % ne;
+          for (int pt = lane; pt < a.p_parts; pt += 64) s[u] += p[pt * a.p_stride];
+        }
+      }
+      int e;
+      const double tot = wave_sum_spread<4>(s, &e);
+      if (e >= 0 && t0 + e < Q * ne) pv[t0 + e] = tot;
     }
     constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
     typedef TV vec_t __attribute__((ext_vector_type(VW)));
@@ -483,19 +492,22 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
     for (int i = 0; i < Q; ++i)
       for (int j = 0; j < Q; ++j)
         Mm[i * Q + j] = 0.5 * (i >= j ? T[i * Q + j] : T[j * Q + i]);
+    // reciprocal diagonal once: the two solves below multiply by it
+    double rd[Q];
+    for (int i = 0; i < Q; ++i) rd[i] = 1.0 / lbr[i * Q + i];
     // Y = L^-T Mm : solve L^T Y = Mm (upper, back substitution per column)
     for (int j = 0; j < Q; ++j)
       for (int i = Q - 1; i >= 0; --i) {
         double s = Mm[i * Q + j];
         for (int k = i + 1; k < Q; ++k) s -= lbr[k * Q + i] * Y[k * Q + j];
-        Y[i * Q + j] = s / lbr[i * Q + i];
+        Y[i * Q + j] = s * rd[i];
       }
     // gA = Y L^-1 : solve gA L = Y, i.e. for each row r: x L = y
     for (int r = 0; r < Q; ++r)
       for (int j = Q - 1; j >= 0; --j) {
         double s = Y[r * Q + j];
         for (int k = j + 1; k < Q; ++k) s -= gA[r * Q + k] * lbr[k * Q + j];
-        gA[r * Q + j] = s / lbr[j * Q + j];
+        gA[r * Q + j] = s * rd[j];
       }
   }
   __syncthreads();
