@@ -690,6 +690,10 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
           int opts) {
   const bool save = !(opts & POTRF_NO_SAVE);
   if (!save) max_tries = 0;
+  if ((opts & POTRF_ASYNC) && save) {
+    set_error("potrf: POTRF_ASYNC needs POTRF_NO_SAVE");
+    return BTA_ERR_INVALID;
+  }
   if (n_pad <= 0 || n_pad % LEAF || lda < n_pad || (Linv && ldinv < n_pad)) {
     set_error("potrf: n_pad must be a positive multiple of 64");
     return BTA_ERR_INVALID;
@@ -730,6 +734,19 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
                                  NB, num_cus(), s);
       if (rc) return rc;
     }
+    if (opts & POTRF_ASYNC) {  // the caller reads info after its own synchronisation
+      dim3 g((unsigned)((n_pad + 255) / 256 > 64 ? 64 : (n_pad + 255) / 256), (unsigned)n_pad);
+      if (!(opts & POTRF_KEEP_UPPER)) {
+        hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, A, lda, n_pad);
+        BTA_LAUNCH_CHECK("zero_upper_kernel");
+      }
+      if (Linv && !(opts & POTRF_BAND_ZERO)) {
+        hipLaunchKernelGGL(trinv_diag_kernel, dim3((unsigned)NB), dim3(256), 0, s, A, lda, Linv,
+                           ldinv, n_pad);
+        BTA_LAUNCH_CHECK("trinv_diag_kernel");
+      }
+      return BTA_OK;
+    }
     int hinfo[2];
     BTA_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s));
     BTA_HIP(hipStreamSynchronize(s));
@@ -755,6 +772,10 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
   set_error("potrf: matrix not positive definite after adding jitter up to %.1e",
             jitter0 * pow(10.0, (double)(max_tries - 1)));
   return BTA_ERR_NOT_PSD;
+}
+
+const int* potrf_status(const void* ws, int64_t n_pad) {
+  return (const int*)((const double*)ws + n_pad * n_pad + LEAF * LEAF);
 }
 
 size_t trtri_ws_bytes(int64_t n_pad) { return (size_t)(n_pad * n_pad / 4 + 64) * sizeof(double); }

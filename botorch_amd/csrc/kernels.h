@@ -29,10 +29,17 @@ size_t potrf_ws_bytes(int64_t n_pad);
 // the diagonal blocks (enough for trtri_lower and for products on 128-row
 // tiles, at any block offset, that treat Linv as lower triangular), and the
 // diagonal blocks are inverted inside the factorisation steps.
-constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2, POTRF_BAND_ZERO = 4;
+// POTRF_ASYNC (with POTRF_NO_SAVE) only enqueues the factorisation: no host
+// synchronisation and no jitter retry.  Its outcome is the pair of ints at
+// potrf_status(ws, n_pad) once the stream has passed the factorisation
+// ({0, 0} = factorised; else {failing column + 1, NaN seen}); the caller checks
+// it and discards what it enqueued after a failure.  trtri_lower may reuse the
+// same workspace: its scratch ends before the pair.
+constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2, POTRF_BAND_ZERO = 4, POTRF_ASYNC = 8;
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
           double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
           int opts = 0);
+const int* potrf_status(const void* ws, int64_t n_pad);
 size_t trtri_ws_bytes(int64_t n_pad);
 // TRTRI_UPPER_ZERO: X's strict upper triangle is already zero.
 constexpr int TRTRI_UPPER_ZERO = 1;

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace bta {
@@ -344,27 +345,16 @@ static void launch_trace(int kind, const TraceArgs& ta, int W, hipStream_t st) {
     hipLaunchKernelGGL((mll_trace_kernel<MD, BTA_KERNEL_RBF>), dim3(W), dim3(256), 0, st, ta);
 }
 
-int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
-                   const double* ls_host, double s, double noise, double c, int max_tries,
-                   double* out_host, void* ws, size_t ws_bytes, hipStream_t st) {
-  if (!X || !y || !ls_host || !out_host || n < 2 || d < 1 || d > 32 || !(s > 0) ||
-      !(noise >= 0) || (kind != BTA_KERNEL_MATERN52 && kind != BTA_KERNEL_RBF)) {
-    set_error("mll_value_grad: invalid arguments");
-    return BTA_ERR_INVALID;
-  }
-  const MllWs w(n, d);
-  if (!ws || ws_bytes < w.total) {
-    set_error("mll_value_grad: workspace %zu < %zu", ws_bytes, w.total);
-    return BTA_ERR_WORKSPACE;
-  }
-  char* base = (char*)ws;
-  double* ls = (double*)(base + w.ls);
-  double* center = (double*)(base + w.center);
+// One attempt of the MLL value+grad on the stream: covariance with the given
+// diagonal, factorisation (enqueued only), inversion, residual products, the
+// trace terms and the final sums into out[0 .. MD + 4]; the factorisation's
+// status pair is copied to out[MD + 5].
+static int enqueue_mll(const MllWs& w, int kind, int64_t n, int d, double s, double diag,
+                       const double* y, double c, hipStream_t st, char* base, void* fact,
+                       size_t fact_bytes) {
   double* xs = (double*)(base + w.xs);
   double* L = (double*)(base + w.L);
   double* Linv = (double*)(base + w.Linv);
-  void* fact = base + w.fact;
-  const size_t fact_bytes = std::max(potrf_ws_bytes(w.n_pad), trtri_ws_bytes(w.n_pad));
   double* r = (double*)(base + w.r);
   double* t = (double*)(base + w.t);
   double* lgd = (double*)(base + w.lgd);
@@ -373,27 +363,10 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   double* partial = (double*)(base + w.partial);
   double* out = (double*)(base + w.out);
   const int64_t np = w.n_pad;
-
-  BTA_HIP(hipMemcpyAsync(ls, ls_host, d * sizeof(double), hipMemcpyHostToDevice, st));
-  int rc = scale_inputs(X, n, d, nullptr, ls, center, xs, np, st);
+  int rc = covar_sym_lower(kind, xs, n, d, s, diag, L, np, np, st);
   if (rc) return rc;
-  // psd_safe_cholesky's schedule: no jitter, then jitter0 * 10^(i-1); K is
-  // rebuilt for each attempt instead of being saved
-  const double jitter0 = 1e-8;
-  for (int attempt = 0;; ++attempt) {
-    const double jit = attempt == 0 ? 0.0 : jitter0 * std::pow(10.0, (double)(attempt - 1));
-    rc = covar_sym_lower(kind, xs, n, d, s, noise + jit, L, np, np, st);
-    if (rc) return rc;
-    double used = 0.0;
-    rc = potrf(L, np, np, Linv, np, 0, jitter0, &used, fact, fact_bytes, st,
-               POTRF_NO_SAVE | POTRF_KEEP_UPPER | POTRF_BAND_ZERO);
-    if (rc != BTA_ERR_NOT_PSD) break;
-    if (attempt >= max_tries) {
-      set_error("mll_value_grad: matrix not positive definite after adding jitter up to %.1e",
-                jit);
-      return BTA_ERR_NOT_PSD;
-    }
-  }
+  rc = potrf(L, np, np, Linv, np, 0, 1e-8, nullptr, fact, fact_bytes, st,
+             POTRF_NO_SAVE | POTRF_KEEP_UPPER | POTRF_BAND_ZERO | POTRF_ASYNC);
   if (rc) return rc;
   rc = trtri_lower(L, np, Linv, np, np, fact, fact_bytes, st, TRTRI_UPPER_ZERO);
   if (rc) return rc;
@@ -438,9 +411,63 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
     hipLaunchKernelGGL(mll_finish_kernel<34>, dim3(1), dim3(256), 0, st, t, lgd, alpha, n, partial,
                        W, out);
   BTA_LAUNCH_CHECK("mll_finish_kernel");
-  std::vector<double> h(3 + w.MD + 2);
-  BTA_HIP(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  BTA_HIP(hipStreamSynchronize(st));
+  BTA_HIP(hipMemcpyAsync(out + 3 + w.MD + 2, potrf_status(fact, np), 2 * sizeof(int),
+                         hipMemcpyDeviceToDevice, st));
+  return BTA_OK;
+}
+
+int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
+                   const double* ls_host, double s, double noise, double c, int max_tries,
+                   double* out_host, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (!X || !y || !ls_host || !out_host || n < 2 || d < 1 || d > 32 || !(s > 0) ||
+      !(noise >= 0) || (kind != BTA_KERNEL_MATERN52 && kind != BTA_KERNEL_RBF)) {
+    set_error("mll_value_grad: invalid arguments");
+    return BTA_ERR_INVALID;
+  }
+  const MllWs w(n, d);
+  if (!ws || ws_bytes < w.total) {
+    set_error("mll_value_grad: workspace %zu < %zu", ws_bytes, w.total);
+    return BTA_ERR_WORKSPACE;
+  }
+  char* base = (char*)ws;
+  double* ls = (double*)(base + w.ls);
+  double* center = (double*)(base + w.center);
+  double* xs = (double*)(base + w.xs);
+  void* fact = base + w.fact;
+  const size_t fact_bytes = std::max(potrf_ws_bytes(w.n_pad), trtri_ws_bytes(w.n_pad));
+  double* out = (double*)(base + w.out);
+  const int64_t np = w.n_pad;
+
+  BTA_HIP(hipMemcpyAsync(ls, ls_host, d * sizeof(double), hipMemcpyHostToDevice, st));
+  int rc = scale_inputs(X, n, d, nullptr, ls, center, xs, np, st);
+  if (rc) return rc;
+  // psd_safe_cholesky's schedule: no jitter, then jitter0 * 10^(i-1); K is
+  // rebuilt for each attempt instead of being saved.  The whole call is
+  // enqueued before the factorisation's outcome is known (one host
+  // synchronisation at the end, so the GPU never waits for the host between
+  // the factorisation and the inversion); after a failure the results are
+  // discarded and the next attempt runs.
+  const double jitter0 = 1e-8;
+  std::vector<double> h(3 + w.MD + 2 + 1);
+  for (int attempt = 0;; ++attempt) {
+    const double jit = attempt == 0 ? 0.0 : jitter0 * std::pow(10.0, (double)(attempt - 1));
+    rc = enqueue_mll(w, kind, n, d, s, noise + jit, y, c, st, base, fact, fact_bytes);
+    if (rc) return rc;
+    BTA_HIP(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    BTA_HIP(hipStreamSynchronize(st));
+    int pst[2];
+    std::memcpy(pst, &h[3 + w.MD + 2], sizeof(pst));
+    if (pst[0] == 0) break;
+    if (pst[1]) {
+      set_error("mll_value_grad: covariance contains NaN (failing column %d)", pst[0]);
+      return BTA_ERR_NAN;
+    }
+    if (attempt >= max_tries) {
+      set_error("mll_value_grad: matrix not positive definite after adding jitter up to %.1e",
+                jit);
+      return BTA_ERR_NOT_PSD;
+    }
+  }
   const double LOG2PI = 1.8378770664093454836;
   out_host[0] = -0.5 * (h[0] + 2.0 * h[1] + (double)n * LOG2PI);
   out_host[1] = 0.5 * h[3] / s;
@@ -449,6 +476,7 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   for (int j = 0; j < d; ++j) out_host[4 + j] = 0.5 * h[5 + j] / ls_host[j];
   return BTA_OK;
 }
+
 
 }  // namespace bta
 

@@ -216,6 +216,47 @@ def test_mll_value_and_adjoint(kind):
                                atol=1e-9)
 
 
+def test_mll_nan_input_raises_then_workspace_recovers():
+    """A NaN in the inputs makes the factorisation report NaN (the status pair
+    read after the single synchronisation at the end of the call): NanError,
+    no jitter retries; the next call on the same workspace is exact again."""
+    _say("mll NaN input, then a clean call on the same workspace")
+    import ctypes
+    from botorch_amd import _lib
+    from botorch_amd.exceptions import NanError
+    g = torch.Generator().manual_seed(5)
+    n, d = 256, 4
+    X = torch.rand(n, d, generator=g, dtype=torch.float64)
+    y = torch.sin(4 * X).sum(-1)
+    y = (y - y.mean()) / y.std()
+    ls = torch.full((d,), 0.4, dtype=torch.float64)
+    os_, noise, c = 1.0, 1e-3, 0.0
+    L = _lib.lib()
+    ws = _lib.workspace(L.bta_mll_workspace_size(n, d), DEV)
+    out = (ctypes.c_double * (4 + d))()
+    lsh = (ctypes.c_double * d)(*ls.tolist())
+    Xbad = X.clone()
+    Xbad[17, 2] = float("nan")
+    Xb, yd = Xbad.to(DEV).contiguous(), y.to(DEV).contiguous()
+    rc = L.bta_mll_value_grad(O.MATERN52, _lib.ptr(Xb), _lib.ptr(yd), n, d, lsh, os_, noise, c,
+                              6, out, _lib.ptr(ws), ws.numel(), _lib.stream())
+    with pytest.raises(NanError):
+        _lib.check(rc)
+    lsr = ls.clone().requires_grad_(True)
+    osr = torch.tensor(os_, dtype=torch.float64, requires_grad=True)
+    nr = torch.tensor(noise, dtype=torch.float64, requires_grad=True)
+    cr = torch.tensor(c, dtype=torch.float64, requires_grad=True)
+    ll = O.exact_mll(O.MATERN52, X, y, lsr, osr, nr, cr) * n
+    ll.backward()
+    Xd = X.to(DEV).contiguous()
+    _lib.check(L.bta_mll_value_grad(O.MATERN52, _lib.ptr(Xd), _lib.ptr(yd), n, d, lsh, os_,
+                                    noise, c, 6, out, _lib.ptr(ws), ws.numel(), _lib.stream()))
+    got = list(out)
+    assert got[0] == pytest.approx(ll.item(), rel=1e-9)
+    torch.testing.assert_close(torch.tensor(got[4:], dtype=torch.float64), lsr.grad, rtol=1e-7,
+                               atol=1e-9)
+
+
 def test_mll_full_size_n4096_d16():
     """The MLL workload at its measured size (n = 4096, d = 16, GP-prior draw
     with l = 0.4 plus noise 0.05^2) against the oracle's autograd on the GPU."""
