@@ -16,6 +16,7 @@ namespace bta {
 void set_error(const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 int num_cus();
+double* pinned_host_buffer();  // 512 doubles, per thread; nullptr if unavailable
 
 #define BTA_HIP(call)                                                   \
   do {                                                                  \

@@ -438,7 +438,15 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
   double* out = (double*)(base + w.out);
   const int64_t np = w.n_pad;
 
-  BTA_HIP(hipMemcpyAsync(ls, ls_host, d * sizeof(double), hipMemcpyHostToDevice, st));
+  // small host copies go through this thread's pinned buffer when there is one:
+  // [0, 32) the length scales, [64, 64 + MD + 6) the sums read back
+  double* pin = pinned_host_buffer();
+  if (pin) {
+    std::memcpy(pin, ls_host, d * sizeof(double));
+    BTA_HIP(hipMemcpyAsync(ls, pin, d * sizeof(double), hipMemcpyHostToDevice, st));
+  } else {
+    BTA_HIP(hipMemcpyAsync(ls, ls_host, d * sizeof(double), hipMemcpyHostToDevice, st));
+  }
   int rc = scale_inputs(X, n, d, nullptr, ls, center, xs, np, st);
   if (rc) return rc;
   // psd_safe_cholesky's schedule: no jitter, then jitter0 * 10^(i-1); K is
@@ -453,8 +461,10 @@ int mll_value_grad(int kind, const double* X, const double* y, int64_t n, int d,
     const double jit = attempt == 0 ? 0.0 : jitter0 * std::pow(10.0, (double)(attempt - 1));
     rc = enqueue_mll(w, kind, n, d, s, noise + jit, y, c, st, base, fact, fact_bytes);
     if (rc) return rc;
-    BTA_HIP(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    double* hb = pin ? pin + 64 : h.data();
+    BTA_HIP(hipMemcpyAsync(hb, out, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     BTA_HIP(hipStreamSynchronize(st));
+    if (pin) std::memcpy(h.data(), hb, h.size() * sizeof(double));
     int pst[2];
     std::memcpy(pst, &h[3 + w.MD + 2], sizeof(pst));
     if (pst[0] == 0) break;

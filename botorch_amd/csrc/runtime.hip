@@ -38,6 +38,22 @@ int num_cus() {
   return cache[dev];
 }
 
+// A 4 KiB pinned host buffer per calling thread, allocated on first use and
+// never freed (freeing it from a thread-exit destructor could run after the
+// HIP runtime has shut down), for the small copies of calls that synchronise
+// before they return; nullptr if the allocation fails (callers then copy
+// through pageable memory).
+double* pinned_host_buffer() {
+  static thread_local double* p = nullptr;
+  static thread_local bool tried = false;
+  if (!tried) {
+    tried = true;
+    void* v = nullptr;
+    if (hipHostMalloc(&v, 4096, hipHostMallocDefault) == hipSuccess) p = (double*)v;
+  }
+  return p;
+}
+
 }  // namespace bta
 
 extern "C" const char* bta_version(void) { return "botorch_amd 0.1.0 (gfx950)"; }
