@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   double* pv = sm + lay.pv;  // Q x (n_b + 1): P[r0 + i][j]
   double* vv = sm + lay.vv;  // packed lower triangle of V_b V_b^T
   {
-    // four tasks per wave and round, their loads issued together
+    // four tasks per wave per pass, their loads issued together
     const int ne = n_b + 1;
     for (int t0 = wv * 4; t0 < Q * ne; t0 += 16) {
       double s[4];
