@@ -100,7 +100,7 @@ def test_qlognei_small_value_grad_fp64():
     assert _rel(Xr.grad.cpu(), g_ref) < 1e-6
 
 
-@pytest.mark.parametrize("q", [2, 4])
+@pytest.mark.parametrize("q", [2, 4, 5, 6, 7])
 def test_qlognei_batch_q(q):
     _say(f"qlognei n=256 d=6 q={q} fp64")
     g = torch.Generator().manual_seed(21 + q)
