@@ -71,6 +71,8 @@ struct H2Args {
                             // segments of seg_chunks x 32 K
 };
 int gemm_h2(const H2Args& a, hipStream_t s);
+// The tile width and worker count gemm_h2 would launch with (after its checks).
+int gemm_h2_plan(const H2Args& a, int* bn, int* workers);
 int split_f16(bool f64, const void* X, int64_t ldx, int64_t rows, int64_t cols,
               int64_t rows_pad, int64_t cols_pad, float scale, bool transpose, _Float16* hi,
               _Float16* lo, int64_t ldo, hipStream_t s);

@@ -30,6 +30,7 @@
 // acquisition/monte_carlo.py:268-290.
 #include "bta_common.h"
 #include "gemm.h"
+#include "gemm_h2_body.h"
 #include "kernels.h"
 
 namespace bta {
@@ -187,11 +188,10 @@ __device__ bool small_chol(const double* A, double jit, double* L) {
   return true;
 }
 
+// Restart b's MC stage and its adjoint, with sm the McLayout area in LDS.
 template <int Q, typename TV>
-__global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
-  extern __shared__ double sm[];
+__device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm) {
   const McLayout lay(Q, a.n_b, a.S);
-  const int b = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int d = a.d, n_b = a.n_b, S = a.S, m = n_b + Q;
   const double ys = a.y_std, ys2 = a.y_std * a.y_std;
@@ -579,6 +579,52 @@ __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   }
 }
 
+template <int Q, typename TV>
+__global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
+  extern __shared__ double sm_dyn[];
+  mc_body<Q, TV>(a, blockIdx.x, sm_dyn);
+}
+
+// One launch for the adjoint GEMM U = V L^-1 (workgroups 0 .. W-1, split-FP16,
+// 64-column tiles) and the MC stage (workgroups W .. W+b-1, one per restart):
+// both need only V, and the MC workgroups fill the CUs as the GEMM's light
+// tiles finish, with no hand-off between queues.  The MC stage uses the first
+// McLayout bytes of the GEMM's LDS ring.
+constexpr int FUSED_LDS = H2Cfg<64, false>::STAGES * H2Cfg<64, false>::STAGE;
+
+template <int Q, typename TV>
+__global__ __launch_bounds__(256, 2) void adjoint_mc_kernel(H2Args g, int W, McArgs a) {
+  __shared__ __attribute__((aligned(1024))) unsigned char lds[FUSED_LDS];
+  if ((int)blockIdx.x < W)
+    gemm_h2_body<64, false>(g, blockIdx.x, W, lds);
+  else
+    mc_body<Q, TV>(a, (int)blockIdx.x - W, (double*)lds);
+}
+
+// Launches the fused kernel when it applies (q <= 5, mixed sample arithmetic,
+// the GEMM planned with 64-column tiles, the MC layout within the ring);
+// returns -1 (nothing launched) otherwise.
+static int launch_adjoint_mc(const H2Args& g, const McArgs& a, int q, int64_t b, bool f32,
+                             hipStream_t s) {
+  int bn = 0, W = 0;
+  const int rc = gemm_h2_plan(g, &bn, &W);
+  if (rc) return rc;
+  const McLayout lay(q, a.n_b, a.S);
+  if (!f32 || q > 5 || bn != 64 || g.seg_chunks > 0 ||
+      (size_t)lay.total * sizeof(double) > (size_t)FUSED_LDS || b < 1 || b > (1 << 24))
+    return -1;
+  const dim3 grid((unsigned)(W + b));
+  switch (q) {
+    case 1: hipLaunchKernelGGL((adjoint_mc_kernel<1, float>), grid, dim3(256), 0, s, g, W, a); break;
+    case 2: hipLaunchKernelGGL((adjoint_mc_kernel<2, float>), grid, dim3(256), 0, s, g, W, a); break;
+    case 3: hipLaunchKernelGGL((adjoint_mc_kernel<3, float>), grid, dim3(256), 0, s, g, W, a); break;
+    case 4: hipLaunchKernelGGL((adjoint_mc_kernel<4, float>), grid, dim3(256), 0, s, g, W, a); break;
+    default: hipLaunchKernelGGL((adjoint_mc_kernel<5, float>), grid, dim3(256), 0, s, g, W, a); break;
+  }
+  BTA_LAUNCH_CHECK("adjoint_mc_kernel");
+  return BTA_OK;
+}
+
 template <int Q>
 static int launch_mc_q(const McArgs& a, int64_t b, bool f32, hipStream_t s) {
   const McLayout lay(Q, a.n_b, a.S);
@@ -734,33 +780,30 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   }
   if (rc) return rc;
   // U = V L^-1 = K_x K^-1 into kx (K x N view of L^-1: rows k >= column n)
-  if (grad) {
-    if (split) {
-      H2Args h{};
-      h.Ah = vh;
-      h.Al = vh + plane;
-      h.Bh = lh + 2 * lplane;
-      h.Bl = lh + 3 * lplane;
-      h.C = (float*)kx;
-      h.Ch = nullptr;
-      h.Cl = nullptr;
-      h.a_row_scale = nullptr;
-      h.a_scale = v_scale;
-      h.b_scale = st->linv_scale;
-      h.c_scale = 1.0f;
-      h.lda = np;
-      h.ldb = np;
-      h.ldc = np;
-      h.M = (int)w.R_pad;
-      h.N = (int)np;
-      h.K = (int)np;
-      h.flags = BTA_GEMM_B_UPPER;
-      h.alpha = 1.0;
-      rc = gemm_h2(h, s);
-    } else {
-      rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, vt, np, st->linv,
-                np, 0.0, nullptr, 0, kx, np, s);
-    }
+  H2Args hu{};
+  if (grad && split) {
+    hu.Ah = vh;
+    hu.Al = vh + plane;
+    hu.Bh = lh + 2 * lplane;
+    hu.Bl = lh + 3 * lplane;
+    hu.C = (float*)kx;
+    hu.Ch = nullptr;
+    hu.Cl = nullptr;
+    hu.a_row_scale = nullptr;
+    hu.a_scale = v_scale;
+    hu.b_scale = st->linv_scale;
+    hu.c_scale = 1.0f;
+    hu.lda = np;
+    hu.ldb = np;
+    hu.ldc = np;
+    hu.M = (int)w.R_pad;
+    hu.N = (int)np;
+    hu.K = (int)np;
+    hu.flags = BTA_GEMM_B_UPPER;
+    hu.alpha = 1.0;
+  } else if (grad) {
+    rc = gemm(fl | BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, vt, np, st->linv, np,
+              0.0, nullptr, 0, kx, np, s);
     if (rc) return rc;
   }
   McArgs a;
@@ -793,7 +836,16 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
   a.tau_relu = st->tau_relu;
   a.tau_max = st->tau_max;
   a.jitter = st->jitter;
-  rc = launch_mc(a, st->q, b, f32, s);
+  // split mode with the gradient: U and the MC stage in one launch where it
+  // applies, else the GEMM and then the MC kernel
+  rc = grad && split ? launch_adjoint_mc(hu, a, st->q, b, f32, s) : -1;
+  if (rc == -1) {
+    if (grad && split) {
+      rc = gemm_h2(hu, s);
+      if (rc) return rc;
+    }
+    rc = launch_mc(a, st->q, b, f32, s);
+  }
   if (rc || !grad) return rc;
   // dK_x = gq U per restart block, contracted with dK/dxs on the fly
   QsGradArgs g;
