@@ -156,8 +156,10 @@ static void best_workers(const GemmArgs& g, int batch, double eff, double& best,
 // Picks the tile shape and worker count from the cost model above; plans are
 // cached per problem signature.
 static GemmPlan plan_gemm(const GemmArgs& g, int batch) {
-  static std::mutex mu;
-  static std::map<std::array<int64_t, 6>, GemmPlan> cache;
+  // Created once and never destroyed: no destructor of this library runs at
+  // process exit, when the HIP runtime or a profiler may already be torn down.
+  static std::mutex& mu = *new std::mutex;
+  static auto& cache = *new std::map<std::array<int64_t, 6>, GemmPlan>;
   const bool f32 = g.flags & BTA_GEMM_F32;
   std::array<int64_t, 6> key = {g.M, g.N, g.K, g.flags, batch, num_cus()};
   {

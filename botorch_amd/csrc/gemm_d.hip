@@ -208,8 +208,9 @@ static int d_split_candidates(const GemmArgs& g, int BN) {
 }
 
 static DPlan d_plan(const GemmArgs& g, int batch) {
-  static std::mutex mu;
-  static std::map<std::array<int64_t, 7>, DPlan> cache;
+  // never destroyed (no exit-time destructor; see plan_gemm)
+  static std::mutex& mu = *new std::mutex;
+  static auto& cache = *new std::map<std::array<int64_t, 7>, DPlan>;
   const int ncu = num_cus();
   const std::array<int64_t, 7> key = {g.M, g.N, g.K, g.flags, batch, ncu, g.C0 != nullptr};
   {

@@ -76,8 +76,9 @@ struct H2Plan {
 };
 
 static H2Plan h2_plan(const H2Args& g, bool acc64) {
-  static std::mutex mu;
-  static std::map<std::tuple<int, int, int, int, int>, H2Plan> cache;
+  // never destroyed (no exit-time destructor; see plan_gemm)
+  static std::mutex& mu = *new std::mutex;
+  static auto& cache = *new std::map<std::tuple<int, int, int, int, int>, H2Plan>;
   const auto key = std::make_tuple(g.M, g.N, g.K, g.flags, (int)acc64);
   {
     std::lock_guard<std::mutex> lk(mu);

@@ -33,10 +33,16 @@
 #include "gemm_h2_body.h"
 #include "kernels.h"
 
+#include <cstdlib>
+
 namespace bta {
 
 constexpr int MC_MAXD = 32;
 constexpr int MC_MAXNB_SLOTS = 4;  // n_b <= 256
+// LDS bound of the MC stage, the same for the standalone and the fused
+// launches so that a state is accepted by both or by neither (the value-only
+// call runs the standalone kernel, value+grad may run the fused one).
+constexpr size_t MC_LDS_MAX = 64 * 1024;
 
 struct McArgs {
   const void* vt;
@@ -601,8 +607,17 @@ __global__ __launch_bounds__(256, 2) void adjoint_mc_kernel(H2Args g, int W, McA
     mc_body<Q, TV>(a, (int)blockIdx.x - W, (double*)lds);
 }
 
+// BTA_SEQUENTIAL_ADJOINT=1 in the environment makes every call take the
+// sequential path (adjoint GEMM, then the MC kernel): the fused launch runs the
+// same two bodies on the same V, so both paths give identical results, which
+// is how the fused launch is tested.
+static bool fused_disabled() {
+  const char* e = getenv("BTA_SEQUENTIAL_ADJOINT");
+  return e && e[0] == '1';
+}
+
 // Launches the fused kernel when it applies (q <= 5, mixed sample arithmetic,
-// the GEMM planned with 64-column tiles, the MC layout within the ring);
+// the GEMM planned with 64-column tiles, the MC layout within MC_LDS_MAX);
 // returns -1 (nothing launched) otherwise.
 static int launch_adjoint_mc(const H2Args& g, const McArgs& a, int q, int64_t b, bool f32,
                              hipStream_t s) {
@@ -610,8 +625,9 @@ static int launch_adjoint_mc(const H2Args& g, const McArgs& a, int q, int64_t b,
   const int rc = gemm_h2_plan(g, &bn, &W);
   if (rc) return rc;
   const McLayout lay(q, a.n_b, a.S);
-  if (!f32 || q > 5 || bn != 64 || g.seg_chunks > 0 ||
-      (size_t)lay.total * sizeof(double) > (size_t)FUSED_LDS || b < 1 || b > (1 << 24))
+  static_assert(MC_LDS_MAX <= (size_t)FUSED_LDS, "MC layout must fit the fused ring");
+  if (!f32 || q > 5 || bn != 64 || g.seg_chunks > 0 || fused_disabled() ||
+      (size_t)lay.total * sizeof(double) > MC_LDS_MAX || b < 1 || b > (1 << 24))
     return -1;
   const dim3 grid((unsigned)(W + b));
   switch (q) {
@@ -629,7 +645,7 @@ template <int Q>
 static int launch_mc_q(const McArgs& a, int64_t b, bool f32, hipStream_t s) {
   const McLayout lay(Q, a.n_b, a.S);
   const size_t bytes = (size_t)lay.total * sizeof(double);
-  if (bytes > 64 * 1024) {
+  if (bytes > MC_LDS_MAX) {
     set_error("qlognei: LDS need %zu bytes exceeds 64 KiB (q=%d n_b=%d S=%d)", bytes, Q, a.n_b,
               a.S);
     return BTA_ERR_UNSUPPORTED;

@@ -23,7 +23,7 @@ int hip_fail(hipError_t e, const char* what) {
 
 // Compute units of the current device (cached per device id).
 int num_cus() {
-  static std::mutex mu;
+  static std::mutex& mu = *new std::mutex;  // never destroyed
   static int cache[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;

@@ -90,6 +90,13 @@ class LogNormalPrior:
         return torch.exp(self.loc + self.scale * z)
 
 
+def _raw_zero_value(c: GreaterThan) -> float:
+    """The value gpytorch starts a softplus-constrained parameter at: its raw
+    parameter is initialised to zero, so the value is lower_bound + softplus(0)
+    = lower_bound + ln 2."""
+    return float(c.transform_value(torch.zeros((), dtype=torch.float64)))
+
+
 class _HyperModule(torch.nn.Module):
     """A module with constrained scalar / vector hyperparameters."""
 
@@ -136,9 +143,8 @@ class _ARDKernel(_HyperModule):
         c = lengthscale_constraint or Positive()
         init = c.initial_value
         if init is None:
-            init = math.log(2.0)  # gpytorch's default raw 0 -> softplus(0)
-            if c.transform is None:
-                init = max(1.0, c.lower_bound)
+            init = 1.0 if c.transform is None else _raw_zero_value(c)
+            init = max(init, c.lower_bound)
         self._register("lengthscale", torch.full((self.ard_num_dims,), float(init),
                                                   dtype=torch.float64), c, lengthscale_prior)
 
@@ -190,7 +196,9 @@ class ScaleKernel(_HyperModule):
         super().__init__()
         self.base_kernel = base_kernel
         c = outputscale_constraint or Positive()
-        init = c.initial_value if c.initial_value is not None else 1.0
+        init = c.initial_value
+        if init is None:
+            init = max(1.0, c.lower_bound) if c.transform is None else _raw_zero_value(c)
         self._register("outputscale", torch.tensor(float(init), dtype=torch.float64), c,
                        outputscale_prior)
 
@@ -215,8 +223,9 @@ class GaussianLikelihood(_HyperModule):
     def __init__(self, noise_prior=None, noise_constraint: GreaterThan | None = None, **kwargs):
         super().__init__()
         c = noise_constraint or GreaterThan(1e-4)
-        init = c.initial_value if c.initial_value is not None else (
-            max(1e-4, 0.6931471805599453) if c.transform else 1e-4)
+        init = c.initial_value
+        if init is None:
+            init = c.lower_bound if c.transform is None else _raw_zero_value(c)
         self._register("noise", torch.tensor(float(init), dtype=torch.float64), c, noise_prior)
 
     @property

@@ -46,8 +46,9 @@ def test_hip_matches_fixture():
     base.lengthscale = w["lengthscale"]
     lik = B.GaussianLikelihood()
     lik.noise = w["noise"]
-    gp = B.SingleTaskGP(w["X"].to(dev), w["Y"].to(dev), likelihood=lik,
-                        covar_module=B.ScaleKernel(base))
+    covar = B.ScaleKernel(base)
+    covar.outputscale = w["outputscale"]
+    gp = B.SingleTaskGP(w["X"].to(dev), w["Y"].to(dev), likelihood=lik, covar_module=covar)
     Xb = w["X"][fx["baseline_idx"]].to(dev)
     acq = B.qLogNoisyExpectedImprovement(
         gp, Xb, sampler=B.SobolQMCNormalSampler(torch.Size([w["S"]]), seed=fx["sampler_seed"]),

@@ -63,8 +63,9 @@ def test_optimize_acqf_qlognei_end_to_end():
     base.lengthscale = w["lengthscale"]
     lik = B.GaussianLikelihood()
     lik.noise = w["noise"]
-    gp = B.SingleTaskGP(w["X"].to(DEV), w["Y"].to(DEV), likelihood=lik,
-                        covar_module=B.ScaleKernel(base))
+    covar = B.ScaleKernel(base)
+    covar.outputscale = w["outputscale"]
+    gp = B.SingleTaskGP(w["X"].to(DEV), w["Y"].to(DEV), likelihood=lik, covar_module=covar)
     acq = B.qLogNoisyExpectedImprovement(gp, w["X"].to(DEV),
                                          sampler=B.SobolQMCNormalSampler(torch.Size([128]),
                                                                          seed=0))

@@ -126,7 +126,9 @@ def _acq_pair(n, d, q, S, precision, seed):
     base.lengthscale = ls
     lik = B.GaussianLikelihood()
     lik.noise = 1e-4
-    gp = B.SingleTaskGP(X.to(DEV), Y.to(DEV), likelihood=lik, covar_module=B.ScaleKernel(base))
+    covar = B.ScaleKernel(base)
+    covar.outputscale = 1.0
+    gp = B.SingleTaskGP(X.to(DEV), Y.to(DEV), likelihood=lik, covar_module=covar)
     orc = O.OracleGP(X, Y, O.MATERN52, ls, 1.0, 1e-4)
     Xb, _ = O.prune_inferior_points(orc, X, num_samples=2048, seed=123)
     acq = B.qLogNoisyExpectedImprovement(gp, Xb.to(DEV),
@@ -137,10 +139,10 @@ def _acq_pair(n, d, q, S, precision, seed):
     return acq, ref, Xq
 
 
-@pytest.mark.parametrize("q", [1, 2, 3])
+@pytest.mark.parametrize("q", [1, 2, 3, 4, 5, 6])
 def test_qlognei_mixed_q_paths(q):
-    # q = 1, 2 take V_b V_b^T from the GEMM epilogue's row-quad Gram partials,
-    # q = 3 (restarts straddle quads) from V itself
+    # q <= 5 run the adjoint GEMM and the MC stage in one launch
+    # (adjoint_mc_kernel<q, float>), q = 6 in two launches
     _say(f"qlognei mixed n=1024 d=8 q={q} S=256 b=64")
     acq, ref, Xq = _acq_pair(1024, 8, q, 256, "mixed", seed=q)
     Xq = Xq[:64]
@@ -187,3 +189,44 @@ def test_qlognei_step_graph_replay(precision):
         v_e, g_e, st_e = acq._launch(Xn.clone(), True)
         torch.cuda.synchronize()
         assert torch.equal(v_g, v_e) and torch.equal(g_g, g_e) and torch.equal(st_g, st_e)
+
+
+def test_fused_adjoint_equals_sequential_path():
+    """adjoint_mc_kernel runs the adjoint GEMM body and the MC body of the
+    sequential path (gemm_h2_kernel, then qlognei_mc_kernel) on the same V: the
+    two paths give identical values, gradients and status words."""
+    import os
+    _say("fused vs sequential adjoint n=2048 d=12 q=4 S=512 b=128 mixed")
+    acq, _, Xq = _acq_pair(2048, 12, 4, 512, "mixed", seed=0)
+    Xd = Xq.to(DEV).contiguous()
+    v_f, g_f, s_f = acq._launch(Xd, True)
+    torch.cuda.synchronize()
+    os.environ["BTA_SEQUENTIAL_ADJOINT"] = "1"
+    try:
+        v_s, g_s, s_s = acq._launch(Xd, True)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["BTA_SEQUENTIAL_ADJOINT"]
+    assert torch.equal(v_f, v_s) and torch.equal(g_f, g_s) and torch.equal(s_f, s_s)
+
+
+@pytest.mark.parametrize("S,fits", [(3700, True), (4096, False)])
+def test_mc_lds_limit_same_for_value_only_and_value_grad(S, fits):
+    """One LDS bound for the MC stage in both launches: a state inside it runs
+    value-only and value+grad with equal values, one beyond it is refused by
+    both (q = 1 with 4096 samples needs ~69 KiB, more than the 64 KiB bound)."""
+    _say(f"mc LDS bound mixed n=512 d=4 q=1 S={S} fits={fits}")
+    acq, ref, Xq = _acq_pair(512, 4, 1, S, "mixed", seed=3)
+    Xd = Xq[:32].to(DEV).contiguous()
+    if fits:
+        v1, _, _ = acq._launch(Xd, False)
+        v2, _, _ = acq._launch(Xd, True)
+        torch.cuda.synchronize()
+        assert torch.equal(v1, v2)
+        with torch.no_grad():
+            v_ref = ref(Xq[:32])
+        assert ((v1.cpu() - v_ref).abs().max() / v_ref.abs().max()).item() < 1e-4
+    else:
+        for need_grad in (False, True):
+            with pytest.raises(B.UnsupportedError):
+                acq._launch(Xd, need_grad)

@@ -133,10 +133,38 @@ def test_sobol_normal_samples_reproducible_and_normal():
     assert torch.isfinite(a).all()
 
 
-def test_cached_root_base_samples_split_rule():
-    Z = O.cached_root_base_samples(64, 3, 2, seed=5)
-    torch.testing.assert_close(Z[:, :3], O.draw_sobol_normal_samples(3, 64, seed=5))
-    torch.testing.assert_close(Z[:, 3:], O.draw_sobol_normal_samples(5, 64, seed=5)[:, 3:])
+def _joint_baseline_columns(acq, Xq, Z):
+    """Samples at the baseline columns of the joint (X_baseline, X) posterior,
+    drawn with the full joint Cholesky (no cached root) from base samples Z."""
+    b = Xq.shape[0]
+    X_full = torch.cat([acq.X_baseline.expand(b, -1, -1), Xq], dim=-2)
+    mean, cov = acq.gp.posterior(X_full)
+    L = torch.linalg.cholesky(cov)
+    s = torch.einsum("bij,sj->sbi", L, Z) + mean.unsqueeze(0)
+    return s[..., : acq.X_baseline.shape[-2]]
+
+
+def test_cached_root_base_samples_keep_baseline_samples():
+    """test/utils/test_low_rank.py:160-175: adding the q new points and their
+    base samples does not change the posterior samples at the baseline points,
+    i.e. the joint samples at the baseline columns equal the baseline samples
+    drawn when the acquisition function was built (logei.py:431-447).  This fixes
+    where the baseline columns of the joint base samples come from; a joint
+    Sobol draw over all n_b + q columns (the mistake planted below) fails it.
+    Where the q new columns come from (the sampler's own Sobol(n_b + q) draw,
+    sampling/normal.py:128-131,182-213) is a convention no fixture pins."""
+    X, Y, ls = _data(14, 3, seed=13)
+    gp = O.OracleGP(X, Y, O.MATERN52, ls, 1.0, 1e-3)
+    acq = O.OracleQLogNEI(gp, X[[1, 4, 9]], num_samples=64, seed=5)
+    g = torch.Generator().manual_seed(4)
+    Xq = torch.rand(5, 2, 3, generator=g, dtype=D)
+    Z = acq.base_samples(2)
+    got = _joint_baseline_columns(acq, Xq, Z)
+    want = acq.baseline_samples.unsqueeze(1).expand_as(got)
+    torch.testing.assert_close(got, want, rtol=1e-9, atol=1e-10)
+    planted = O.draw_sobol_normal_samples(5, 64, seed=5)  # one joint draw: wrong source
+    bad = _joint_baseline_columns(acq, Xq, planted)
+    assert (bad - want).abs().max() > 1e-2
 
 
 # ---------------------------------------------------------------- cached sampling
@@ -168,6 +196,34 @@ def test_pareto_properties():
     assert p.item() == 1.0 and g.item() == pytest.approx(-1.0) and h.item() == pytest.approx(1.0)
     big = torch.tensor([1e4], dtype=D)
     assert (O.pareto(big) * big**2).item() == pytest.approx(2.0, rel=1e-3)
+
+
+def test_fatplus_value_at_zero_and_closed_form_derivative():
+    """fatplus(x, tau) = tau (softplus(x / tau) + ALPHA cauchy(x / tau)) with
+    ALPHA = 0.1 (utils/safe_math.py:322, csrc/logei_fused.cpp:28): at x = 0 the
+    value is tau (ln 2 + ALPHA), which fixes the weight.  The derivative of
+    log_fatplus is checked against the reference's closed form
+    (csrc/logei_fused.cpp:97-111): (sigmoid(y) - 2 ALPHA y c(y)^2) / (tau f),
+    y = x / tau, c(y) = 1 / (1 + y^2), f = softplus(y) + ALPHA c(y)."""
+    for tau in (1e-6, 1e-2, 0.7):
+        v = O.fatplus(torch.tensor([0.0], dtype=D), tau).item()
+        assert v == pytest.approx(tau * (math.log(2.0) + 0.1), rel=1e-15)
+    tau = 0.05
+    x = torch.linspace(-3, 3, 241, dtype=D).requires_grad_(True)
+    (g,) = torch.autograd.grad(O.log_fatplus(x, tau).sum(), x)
+    y = x.detach() / tau
+    c = 1.0 / (1.0 + y * y)
+    f = torch.nn.functional.softplus(y) + 0.1 * c
+    closed = (torch.sigmoid(y) - 2 * 0.1 * y * c * c) / (tau * f)
+    # above y = 20 torch's softplus (which the reference calls) is linear, so its
+    # derivative there is 1 instead of sigmoid(y) = 1 - O(e^-20): 2e-9 relative
+    inner = y.abs() <= 20
+    torch.testing.assert_close(g[inner], closed[inner], rtol=1e-12, atol=1e-12)
+    torch.testing.assert_close(g, closed, rtol=3e-9, atol=1e-12)
+    # a weight of 0.2 instead of 0.1 would move the derivative by ~1e-1 near 0
+    f2 = torch.nn.functional.softplus(y) + 0.2 * c
+    wrong = (torch.sigmoid(y) - 2 * 0.2 * y * c * c) / (tau * f2)
+    assert (g - wrong).abs().max() > 1e-2 * g.abs().max()
 
 
 def test_fatplus_bounds_and_asymptotics():
