@@ -203,6 +203,59 @@ def cpu_baseline_headline(w, Xb, budget_s=15.0):
                       f"({sum(times):.1f}s), median {med * 1e3:.1f} ms per step"}
 
 
+def bench_batched(steps=20, warmup=3):
+    """The batched configuration: 64 uneven-n Matern-5/2 GPs, each scoring 256
+    restarts x q = 2 with 256 samples (value + grad, mixed), as one batched step
+    (five launches for all models), that step replayed from a HIP graph, and the
+    same models stepped one after another on one stream."""
+    import botorch_amd as B
+    from botorch_amd import workloads
+    ws = workloads.gp_batch()
+    acqs = []
+    for w in ws:
+        d = w["X"].shape[-1]
+        base = B.MaternKernel(ard_num_dims=d)
+        base.lengthscale = w["lengthscale"]
+        covar = B.ScaleKernel(base)
+        covar.outputscale = w["outputscale"]
+        lik = B.GaussianLikelihood()
+        lik.noise = w["noise"]
+        gp = B.SingleTaskGP(w["X"].to("cuda"), w["Y"].to("cuda"), likelihood=lik,
+                            covar_module=covar)
+        with torch.no_grad():
+            Xb = B.prune_inferior_points(
+                gp, w["X"].to("cuda"),
+                sampler=B.SobolQMCNormalSampler(torch.Size([2048]), seed=123))
+        acqs.append(B.qLogNoisyExpectedImprovement(
+            gp, Xb, sampler=B.SobolQMCNormalSampler(torch.Size([w["S"]]), seed=0),
+            prune_baseline=False, precision="mixed"))
+    Xs = [w["Xq"].to("cuda").contiguous() for w in ws]
+    batch = B.BatchedQLogNEI(acqs, 256, q=2)
+    for buf, x in zip(batch.X, Xs):
+        buf.copy_(x)
+    t_batch = _timed(batch.launch, steps, warmup)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        batch.launch()
+    t_graph = _timed(graph.replay, steps, warmup)
+
+    def sequential():
+        for acq, x in zip(acqs, Xs):
+            acq._launch(x, True)
+
+    t_seq = _timed(sequential, max(5, steps // 4), 2)
+    ns = [w["X"].shape[0] for w in ws]
+    nb = sum(len(w["Xq"]) for w in ws)
+    log(f"batched 64 models: {t_batch * 1e3:.3f} ms/step (graph {t_graph * 1e3:.3f}), "
+        f"one model after another {t_seq * 1e3:.3f} ms")
+    return {"workload": "64 Matern-5/2 GPs, n log-uniform in [128, 4096] (here "
+                        f"{min(ns)}..{max(ns)}), d in 4..20, 256 restarts x q=2 x S=256 each, "
+                        "value+grad, mixed",
+            "models": len(ws), "ms_per_step": t_batch * 1e3, "graph_ms_per_step": t_graph * 1e3,
+            "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t_batch,
+            "restart_evals_per_s": nb / t_batch}
+
+
 def build_mll(n=4096, d=16):
     from botorch_amd import workloads
     w = workloads.gp_mll(n=n, d=d)
@@ -221,6 +274,7 @@ def main():
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-mll", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="launch only a copy and the forward contraction (counter passes)")
     args = ap.parse_args()
@@ -354,6 +408,11 @@ def main():
                                           / g_ref_m[1:].abs().max()).item()),
                "rel_err_loglik": float(abs(g_hip[0] - g_ref_m[0]) / abs(g_ref_m[0])),
                "flops": nm**3 / 3 + 2 * nm**3 / 3 + nm * nm * (5 * dm + 18) / 2}
+    # ---------------- batched uneven models -------------------------------------------
+    batched = None
+    if not args.no_batch:
+        log("batched uneven models")
+        batched = bench_batched()
     # ---------------- CPU baseline --------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -379,7 +438,7 @@ def main():
         "speedup_fp64hip_vs_fp64_gpu": (t_torch / t_fp64) if t_torch else None,
         "step_tflops_mixed": step_flops / t_mixed / 1e12,
         "accuracy_max_rel_err_vs_fp64_oracle": acc,
-        "roofline": roof, "cpu_baseline": cpu, "mll": mll,
+        "roofline": roof, "cpu_baseline": cpu, "mll": mll, "batched_uneven": batched,
     }
     print(json.dumps(res))
 

@@ -5,9 +5,10 @@ qLogNoisyExpectedImprovement.forward, optimize_acqf, fit_gpytorch_mll) over
 hand-written HIP kernels for gfx950 in libbotorch_amd.so (C ABI:
 include/botorch_amd.h).
 """
-from .acquisition import TAU_MAX, TAU_RELU, prune_inferior_points, qLogNoisyExpectedImprovement
-from .exceptions import (BotorchError, InputDataError, NanError, NotPSDError,
-                         OptimizationGradientError, UnsupportedError)
+from .acquisition import (TAU_MAX, TAU_RELU, BatchedQLogNEI, prune_inferior_points,
+                          qLogNoisyExpectedImprovement)
+from .exceptions import (BotorchError, BotorchWarning, InputDataError, NanError, NotPSDError,
+                         NumericalWarning, OptimizationGradientError, UnsupportedError)
 from .gp import (ConstantMean, GaussianLikelihood, GPyTorchPosterior, GreaterThan,
                  LogNormalPrior, MaternKernel, Positive, RBFKernel, ScaleKernel, SingleTaskGP,
                  Standardize, psd_safe_cholesky)

@@ -60,6 +60,7 @@ class GpModel(ctypes.Structure):
         ("train_xs", _p), ("center", _p), ("lengthscale", _p),
         ("outputscale", _f64), ("noise", _f64), ("mean_constant", _f64),
         ("y_mean", _f64), ("y_std", _f64), ("linv", _p), ("alpha", _p),
+        ("chol_jitter", _f64), ("refine_steps", _i32),
     ]
 
 
@@ -70,7 +71,25 @@ class QLogNEIState(ctypes.Structure):
         ("xb_s", _p), ("linv", _p), ("ext", _p), ("Lb", _p), ("Zt", _p), ("best_f", _p),
         ("tau_relu", _f64), ("tau_max", _f64), ("max_tries", _i32),
         ("linv_scale", ctypes.c_float), ("jitter", _f64),
+        ("Lb_levels", _p), ("lb_levels_mask", _i32),
     ]
+
+
+class QLogNEIBatchItem(ctypes.Structure):
+    """bta_qlognei_batch_item"""
+    _fields_ = [
+        ("model", ctypes.POINTER(GpModel)), ("state", ctypes.POINTER(QLogNEIState)),
+        ("X", _p), ("b", _i64), ("value", _p), ("grad", _p), ("status", _p),
+    ]
+
+
+class QLogNEIBatchPlan(ctypes.Structure):
+    """bta_qlognei_batch_plan"""
+    _fields_ = [("ws", _p)] + [(f, _i32) for f in (
+        "n_items", "q", "kind", "kd", "grad", "covar_blocks", "mc_blocks", "fin_blocks",
+        "fwd_tiles", "adj_tiles", "fwd_workers", "adj_workers")] + [(f, _sz) for f in (
+            "off_cov", "off_fwd", "off_adj", "off_mc", "off_grad", "off_fin", "off_cov_start",
+            "off_mc_start", "off_fin_start", "off_fwd_tiles", "off_adj_tiles")]
 
 
 # name -> (restype, argtypes); the exact list of extern "C" entry points.
@@ -111,12 +130,16 @@ SIGNATURES = {
     "bta_qlognei_value_grad": (ctypes.c_int, [ctypes.POINTER(GpModel),
                                               ctypes.POINTER(QLogNEIState), _p, _i64, _p, _p,
                                               _p, _p, _sz, _p]),
+    "bta_qlognei_batch_workspace_size": (_sz, [ctypes.POINTER(QLogNEIBatchItem), _i32]),
+    "bta_qlognei_batch_prepare": (ctypes.c_int, [ctypes.POINTER(QLogNEIBatchItem), _i32, _p,
+                                                 _sz, ctypes.POINTER(QLogNEIBatchPlan), _p]),
+    "bta_qlognei_batch_value_grad": (ctypes.c_int, [ctypes.POINTER(QLogNEIBatchPlan), _p]),
     "bta_mll_workspace_size": (_sz, [_i64, _i32]),
     "bta_mll_value_grad": (ctypes.c_int, [_i32, _p, _p, _i64, _i32, ctypes.POINTER(_f64), _f64,
                                           _f64, _f64, _i32, ctypes.POINTER(_f64), _p, _sz, _p]),
 }
 
-ABI_VERSION = 4  # BTA_ABI_VERSION of include/botorch_amd.h
+ABI_VERSION = 5  # BTA_ABI_VERSION of include/botorch_amd.h
 
 _lock = threading.Lock()
 _lib = None

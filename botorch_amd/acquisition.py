@@ -21,7 +21,7 @@ import torch
 from torch import Tensor
 
 from . import _lib
-from .exceptions import NanError, NotPSDError, UnsupportedError
+from .exceptions import BotorchWarning, NanError, NotPSDError, UnsupportedError
 from .gp import SingleTaskGP
 from .sampling import SobolQMCNormalSampler, draw_sobol_normal_samples
 
@@ -178,6 +178,7 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         self._ws = None
         self._baseline_L = Lb
         self._baseline_best_f = best_f
+        self._levels = None  # joint-sampling fallback factors, made on first use
 
     def base_samples_t(self, q: int) -> Tensor:
         """(n_b + q) x S base samples: baseline columns from the baseline sampler,
@@ -200,9 +201,37 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
             self._ws = _lib.workspace(need, self.X_baseline.device)
         return self._ws
 
-    def _launch(self, X: Tensor, need_grad: bool):
+    def _joint_levels(self):
+        """The baseline factors of the joint-sampling fallback: level t =
+        chol(Sigma_bb + j_t I), j_0 = 0, j_t = 1e-8 10^(t-1), for t = 0..max_tries,
+        with Sigma_bb the model's posterior covariance at X_baseline (recomputed,
+        not the cached root), and the mask of the levels whose Cholesky exists."""
+        if getattr(self, "_levels", None) is None:
+            from .gp import psd_safe_cholesky
+            with torch.no_grad():
+                cov = self.model.posterior(self.X_baseline).covariance_matrix
+            if torch.isnan(cov).any():
+                raise NanError("cholesky input contains NaNs")
+            n_b = cov.shape[-1]
+            tries = self._state.max_tries
+            levels = torch.zeros(tries + 1, n_b, n_b, dtype=torch.float64, device=cov.device)
+            mask = 0
+            eye = torch.eye(n_b, dtype=torch.float64, device=cov.device)
+            for t in range(tries + 1):
+                jit = 0.0 if t == 0 else self._state.jitter * 10.0 ** (t - 1)
+                try:
+                    levels[t] = psd_safe_cholesky(cov + jit * eye, max_tries=0)
+                    mask |= 1 << t
+                except NotPSDError:
+                    pass
+            self._levels = (levels, mask)
+        return self._levels
+
+    def _launch(self, X: Tensor, need_grad: bool, joint: bool = False):
         """Enqueue one value(+grad) evaluation; returns device tensors
-        (value b, grad b x q x d or None, status b)."""
+        (value b, grad b x q x d or None, status b).  joint: sample from the
+        Cholesky of the joint (X_baseline, X) posterior instead of the cached
+        baseline root."""
         L = _lib.lib()
         b, q, d = X.shape
         if q > 8:
@@ -216,10 +245,19 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         value = torch.empty(b, dtype=torch.float64, device=Xc.device)
         grad = torch.empty(b, q, d, dtype=torch.float64, device=Xc.device) if need_grad else None
         status = torch.empty(b, dtype=torch.int32, device=Xc.device)
-        _lib.check(L.bta_qlognei_value_grad(ctypes.byref(self._keep["cache"]["model"]),
-                                            ctypes.byref(st), _lib.ptr(Xc), b, _lib.ptr(value),
-                                            _lib.ptr(grad), _lib.ptr(status), _lib.ptr(ws),
-                                            ws.numel(), _lib.stream()), "qlognei_value_grad")
+        if joint:
+            levels, mask = self._joint_levels()
+            st.Lb_levels = _lib.ptr(levels)
+            st.lb_levels_mask = mask
+        try:
+            _lib.check(L.bta_qlognei_value_grad(ctypes.byref(self._keep["cache"]["model"]),
+                                                ctypes.byref(st), _lib.ptr(Xc), b,
+                                                _lib.ptr(value), _lib.ptr(grad),
+                                                _lib.ptr(status), _lib.ptr(ws), ws.numel(),
+                                                _lib.stream()), "qlognei_value_grad")
+        finally:
+            st.Lb_levels = None
+            st.lb_levels_mask = 0
         return value, grad, status
 
     @staticmethod
@@ -237,8 +275,20 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         if jit:
             warnings.warn("A not p.d., added jitter to the diagonal", RuntimeWarning)
 
+    @staticmethod
+    def _cached_root_failed(status: Tensor) -> bool:
+        return bool(((status & (_lib.STATUS_NAN | _lib.STATUS_NOT_PSD)) != 0).any())
+
     def _run(self, X: Tensor, need_grad: bool):
         value, grad, status = self._launch(X, need_grad)
+        if self._cached_root_failed(status):
+            # acquisition/cached_cholesky.py:143-158: the low-rank update raised
+            # NanError / NotPSDError, so the whole batch is sampled again from the
+            # joint posterior's Cholesky
+            warnings.warn("Low-rank cholesky updates failed due NaNs or due to an "
+                          "ill-conditioned covariance matrix. Falling back to standard "
+                          "sampling.", BotorchWarning, stacklevel=3)
+            value, grad, status = self._launch(X, need_grad, joint=True)
         self.check_status(status)
         return value, grad
 
@@ -264,3 +314,90 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         else:
             val, _ = self._run(Xf.detach(), False)
         return val.reshape(batch_shape).to(X.dtype)
+
+
+class BatchedQLogNEI:
+    """qLogNEI value+grad of many independent models in one batched step.
+
+    Stands in for evaluating the acquisition functions of many SingleTaskGPs
+    one after another (one L-BFGS-B step of each model's optimize_acqf,
+    generation/gen.py:423-485): the step over all of them is five launches, one
+    per stage, each covering every model (``bta_qlognei_batch_value_grad``).
+    Each acquisition function keeps its own model, baseline and base samples;
+    the batch shares q, the kernel kind and the mixed precision.  The query
+    points live in fixed device buffers (``X[i]``, restarts[i] x q x d) that the
+    step reads in place, so a captured HIP graph of ``launch`` stays valid when
+    they are refilled.
+    """
+
+    def __init__(self, acqfs, restarts, q: int, need_grad: bool = True):
+        L = _lib.lib()
+        if isinstance(restarts, int):
+            restarts = [restarts] * len(acqfs)
+        if len(acqfs) != len(restarts) or not acqfs:
+            raise ValueError("one restart count per acquisition function")
+        self.acqfs = list(acqfs)
+        self.q = q
+        self.need_grad = need_grad
+        dev = self.acqfs[0].X_baseline.device
+        self.X, self.value, self.grad, self.status = [], [], [], []
+        self._states = []
+        items = (_lib.QLogNEIBatchItem * len(acqfs))()
+        for i, (acq, b) in enumerate(zip(self.acqfs, restarts)):
+            if acq.precision != "mixed":
+                raise UnsupportedError("the batched step runs the mixed precision only")
+            d = acq.X_baseline.shape[-1]
+            self.X.append(torch.zeros(b, q, d, dtype=torch.float64, device=dev))
+            self.value.append(torch.empty(b, dtype=torch.float64, device=dev))
+            self.grad.append(torch.empty(b, q, d, dtype=torch.float64, device=dev)
+                             if need_grad else None)
+            self.status.append(torch.empty(b, dtype=torch.int32, device=dev))
+            st = _lib.QLogNEIState()  # a copy: this batch's q and base samples
+            ctypes.memmove(ctypes.byref(st), ctypes.byref(acq._state), ctypes.sizeof(st))
+            st.q = q
+            st.Zt = _lib.ptr(acq.base_samples_t(q))
+            self._states.append(st)
+            it = items[i]
+            it.model = ctypes.pointer(acq._keep["cache"]["model"])
+            it.state = ctypes.pointer(st)
+            it.X = _lib.ptr(self.X[i])
+            it.b = b
+            it.value = _lib.ptr(self.value[i])
+            it.grad = _lib.ptr(self.grad[i])
+            it.status = _lib.ptr(self.status[i])
+        self._items = items
+        need = L.bta_qlognei_batch_workspace_size(items, len(acqfs))
+        if need == 0:
+            raise ValueError("invalid batch items")
+        self.ws = _lib.workspace(need, dev)
+        self.plan = _lib.QLogNEIBatchPlan()
+        _lib.check(L.bta_qlognei_batch_prepare(items, len(acqfs), _lib.ptr(self.ws),
+                                               self.ws.numel(), ctypes.byref(self.plan),
+                                               _lib.stream()), "qlognei_batch_prepare")
+
+    def launch(self) -> None:
+        """Enqueue one step over every model (reads ``X``, writes ``value``,
+        ``grad``, ``status``); no host synchronisation."""
+        _lib.check(_lib.lib().bta_qlognei_batch_value_grad(ctypes.byref(self.plan),
+                                                           _lib.stream()),
+                   "qlognei_batch_value_grad")
+
+    def value_and_grad(self, Xs):
+        """Copies Xs[i] (restarts[i] x q x d) into the buffers, runs the step and
+        returns ([value_i], [grad_i]), raising like the one-model step on failed
+        samples."""
+        for buf, X in zip(self.X, Xs):
+            buf.copy_(X)
+        self.launch()
+        for i, acq in enumerate(self.acqfs):
+            if qLogNoisyExpectedImprovement._cached_root_failed(self.status[i]):
+                # this model's cached-root update failed: its acquisition function
+                # samples it again from the joint posterior (and warns)
+                v, g = acq._run(self.X[i], self.need_grad)
+                self.value[i].copy_(v)
+                if self.need_grad:
+                    self.grad[i].copy_(g)
+                self.status[i].zero_()
+        qLogNoisyExpectedImprovement.check_status(torch.cat(self.status))
+        return ([v.clone() for v in self.value],
+                [g.clone() for g in self.grad] if self.need_grad else None)

@@ -32,6 +32,7 @@
 #include "gemm_h2_body.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -72,8 +73,24 @@ static double h2_cost(const H2Args& g, int BN, int W) {
 }
 
 struct H2Plan {
-  int bn, workers;
+  int bn, workers, xcd;
 };
+
+// Worker groups for gemm_h2_body's strip dealing: one per XCD (workgroups go
+// to the XCDs round-robin by id; 32 CUs per XCD), when the tile order is strip
+// by strip (triangular B), there are at least two strips per group and the
+// worker count divides evenly.  BTA_H2_XCD=0 in the environment turns it off.
+static int h2_xcd_groups(const H2Args& g, int BN, int W) {
+  static const int off = [] {
+    const char* e = getenv("BTA_H2_XCD");
+    return e && e[0] == '0';
+  }();
+  const int G = num_cus() / 32;
+  if (off || G < 2 || !(g.flags & (BTA_GEMM_B_LOWER | BTA_GEMM_B_UPPER)) || W % G ||
+      g.N / BN < 2 * G)
+    return 1;
+  return G;
+}
 
 static H2Plan h2_plan(const H2Args& g, bool acc64) {
   // never destroyed (no exit-time destructor; see plan_gemm)
@@ -87,7 +104,7 @@ static H2Plan h2_plan(const H2Args& g, bool acc64) {
   }
   const int ncu = num_cus();
   const int slots = acc64 ? ncu : 2 * ncu;  // resident workgroups
-  H2Plan best{64, 1};
+  H2Plan best{64, 1, 1};
   double bt = 1e300;
   for (int bn : {64, 128}) {
     if (g.N % bn) continue;
@@ -97,7 +114,7 @@ static H2Plan h2_plan(const H2Args& g, bool acc64) {
       const double t = h2_cost(g, bn, W);
       if (t < bt) {
         bt = t;
-        best = {bn, W};
+        best = {bn, W, h2_xcd_groups(g, bn, W)};
       }
     }
   }
@@ -119,20 +136,23 @@ static int h2_check(const H2Args& a) {
   return BTA_OK;
 }
 
-int gemm_h2_plan(const H2Args& a, int* bn, int* workers) {
+int gemm_h2_plan(const H2Args& a, int* bn, int* workers, int* xcd_groups) {
   const int rc = h2_check(a);
   if (rc) return rc;
   const H2Plan p = h2_plan(a, a.seg_chunks > 0);
   *bn = p.bn;
   *workers = p.workers;
+  if (xcd_groups) *xcd_groups = p.xcd;
   return BTA_OK;
 }
 
-int gemm_h2(const H2Args& a, hipStream_t s) {
-  const int rc = h2_check(a);
+int gemm_h2(const H2Args& args, hipStream_t s) {
+  const int rc = h2_check(args);
   if (rc) return rc;
-  const bool acc64 = a.seg_chunks > 0;
-  const H2Plan p = h2_plan(a, acc64);
+  const bool acc64 = args.seg_chunks > 0;
+  const H2Plan p = h2_plan(args, acc64);
+  H2Args a = args;
+  a.xcd_groups = p.xcd;
   if (p.bn == 128) {
     if (acc64) hipLaunchKernelGGL((gemm_h2_kernel<128, true>), dim3(p.workers), dim3(256), 0, s, a);
     else hipLaunchKernelGGL((gemm_h2_kernel<128, false>), dim3(p.workers), dim3(256), 0, s, a);

@@ -61,14 +61,13 @@ __device__ __forceinline__ void h2_frag(uint32_t a, half8 (&hi)[F], half8 (&lo)[
   if (F > 3) { hi[3] = h2_read<3072>(a); lo[3] = h2_read<P + 3072>(a); }
 }
 
-// The main loop and epilogue of worker w of W, with the ring in lds
-// (H2Cfg<BN, ACC64>::STAGES * STAGE bytes of LDS, 1 KiB aligned).
+// The main loop and epilogue of one 128 x BN output tile (tm, tn), with the
+// ring in lds (H2Cfg<BN, ACC64>::STAGES * STAGE bytes of LDS, 1 KiB aligned).
 template <int BN, bool ACC64>
-__device__ __forceinline__ void gemm_h2_body(const H2Args& g, int w, int W, unsigned char* lds) {
+__device__ __forceinline__ void gemm_h2_tile(const H2Args& g, const int tm, const int tn,
+                                             unsigned char* lds) {
   using C = H2Cfg<BN, ACC64>;
   constexpr int S = C::STAGES;
-  const int MT = g.M / H2_BM, NT = g.N / BN;
-  const int total = MT * NT;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
   // fragment reads: row l16 of a 16-row block, chunk gq in its swizzled slot
@@ -77,13 +76,7 @@ __device__ __forceinline__ void gemm_h2_body(const H2Args& g, int w, int W, unsi
   const int lrow = lane >> 2;
   const int lchunk = (lane & 3) ^ h2_swizzle(lrow);
   const int seg = ACC64 ? (g.seg_chunks > 0 ? g.seg_chunks : 8) : 0;  // chunks per FP64 add
-
-  for (int jj = 0;; ++jj) {
-    const int t = h2_tile_of(jj, w, W);
-    if (jj * W >= total) break;
-    if (t >= total) continue;
-    int tm, tn;
-    decode_tile(t, g.flags, MT, NT, tm, tn);
+  {
     int kb = 0, ke = g.K;
     if (g.flags & BTA_GEMM_B_LOWER) ke = min(ke, (tn + 1) * BN);
     if (g.flags & BTA_GEMM_B_UPPER) kb = max(kb, tn * BN);
@@ -235,6 +228,58 @@ __device__ __forceinline__ void gemm_h2_body(const H2Args& g, int w, int W, unsi
           }
         }
       }
+  }
+}
+
+// Worker w of W: the tiles h2_tile_of(jj, w, W), jj = 0, 1, ... of the
+// problem's longest-first tile order.  With g.xcd_groups = G > 1 (triangular
+// B, tiles ordered column strip by column strip), the workers form G groups by
+// w % G - the XCD a workgroup is dispatched to - and group x takes the column
+// strips x, x + G, x + 2G, ... of that order, so all row tiles of a strip of
+// B run on one XCD and read the strip through one L2; within the group the
+// tiles are dealt as above over its W / G workers (W a multiple of G).
+template <int BN, bool ACC64>
+__device__ __forceinline__ void gemm_h2_body(const H2Args& g, int w, int W, unsigned char* lds) {
+  const int MT = g.M / H2_BM, NT = g.N / BN;
+  const int G = g.xcd_groups;
+  if (G > 1) {
+    const int x = w % G, lw = w / G, Wx = W / G;
+    const int total = ((NT - x + G - 1) / G) * MT;  // tiles of strips x, x + G, ...
+    for (int jj = 0;; ++jj) {
+      const int u = h2_tile_of(jj, lw, Wx);
+      if (jj * Wx >= total) break;
+      if (u >= total) continue;
+      const int t = ((u / MT) * G + x) * MT + u % MT;
+      int tm, tn;
+      decode_tile(t, g.flags, MT, NT, tm, tn);
+      gemm_h2_tile<BN, ACC64>(g, tm, tn, lds);
+    }
+    return;
+  }
+  const int total = MT * NT;
+  for (int jj = 0;; ++jj) {
+    const int t = h2_tile_of(jj, w, W);
+    if (jj * W >= total) break;
+    if (t >= total) continue;
+    int tm, tn;
+    decode_tile(t, g.flags, MT, NT, tm, tn);
+    gemm_h2_tile<BN, ACC64>(g, tm, tn, lds);
+  }
+}
+
+// Worker w of W over a list of tiles of many problems: tiles[t] = (problem,
+// tm, tn, -), listed longest K range first; problem p's arguments are tab[p].
+template <int BN>
+__device__ __forceinline__ void gemm_h2_list_body(const H2Args* __restrict__ tab,
+                                                  const int4* __restrict__ tiles, int total,
+                                                  int w, int W, unsigned char* lds) {
+  for (int jj = 0;; ++jj) {
+    const int t = h2_tile_of(jj, w, W);
+    if (jj * W >= total) break;
+    if (t >= total) continue;
+    const int4 e = tiles[t];
+    const H2Args g = tab[e.x];
+    gemm_h2_tile<BN, false>(g, e.y, e.z, lds);
   }
 }
 

@@ -76,6 +76,92 @@ __global__ void residual_kernel(const double* y, int64_t n, double c, double* r,
   if (i < n_pad) r[i] = i < n ? y[i] - c : 0.0;
 }
 
+// out[c][i] = Y[c][i] - sum_{j < n} B[c][j] Khat[j][i] for the nc row vectors c of
+// B and Y (row stride ld, i < n_pad; Khat = s k(xs, xs) + diag I on the first n
+// points, zero beyond), the residual of one refinement step of the solves.
+// The kernel values come from direct coordinate differences, evaluated on the
+// fly (nothing n x n is stored).  Block: 64 columns i, its 4 waves split each
+// 64-point tile of j in four; KR_RHS right-hand sides per pass (grid.y).
+constexpr int KR_RHS = 16;
+
+__global__ __launch_bounds__(256) void kmat_residual_kernel(int kind, const double* __restrict__ xs,
+                                                            int64_t n, int d, double s,
+                                                            double diag,
+                                                            const double* __restrict__ B,
+                                                            const double* __restrict__ Y,
+                                                            int64_t ld, int nc, double* out) {
+  __shared__ double xj[64][33];
+  __shared__ double bj[KR_RHS][64];
+  __shared__ double part[4][KR_RHS][65];
+  const int tid = threadIdx.x, il = tid & 63, chunk = tid >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + il;
+  const int c0 = blockIdx.y * KR_RHS;
+  const int ncp = min(KR_RHS, nc - c0);
+  double xi[32];
+#pragma unroll
+  for (int t = 0; t < 32; ++t) xi[t] = (t < d && i < n) ? xs[i * d + t] : 0.0;
+  double acc[KR_RHS];
+#pragma unroll
+  for (int c = 0; c < KR_RHS; ++c) acc[c] = 0.0;
+  for (int64_t j0 = 0; j0 < n; j0 += 64) {
+    __syncthreads();
+    for (int e = tid; e < 64 * d; e += 256) {
+      const int jj = e / d, t = e % d;
+      xj[jj][t] = j0 + jj < n ? xs[(j0 + jj) * d + t] : 0.0;
+    }
+    for (int e = tid; e < KR_RHS * 64; e += 256) {
+      const int c = e / 64, jj = e % 64;
+      bj[c][jj] = (c < ncp && j0 + jj < n) ? B[(int64_t)(c0 + c) * ld + j0 + jj] : 0.0;
+    }
+    __syncthreads();
+    for (int jj = chunk * 16; jj < chunk * 16 + 16; ++jj) {
+      const int64_t j = j0 + jj;
+      if (j >= n) break;
+      double r2 = 0.0;
+#pragma unroll
+      for (int t = 0; t < 32; ++t)
+        if (t < d) {
+          const double df = xi[t] - xj[jj][t];
+          r2 = fma(df, df, r2);
+        }
+      const double k = kern_value(kind, r2, s) + (j == i ? diag : 0.0);
+#pragma unroll
+      for (int c = 0; c < KR_RHS; ++c) acc[c] = fma(bj[c][jj], k, acc[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < KR_RHS; ++c) part[chunk][c][il] = acc[c];
+  __syncthreads();
+  if (chunk == 0) {
+    for (int c = 0; c < ncp; ++c) {
+      const double sum = i < n ? ((part[0][c][il] + part[1][c][il]) +
+                                  (part[2][c][il] + part[3][c][il]))
+                               : 0.0;
+      const int64_t o = (int64_t)(c0 + c) * ld + i;
+      out[o] = Y[o] - sum;
+    }
+  }
+}
+
+// R = Y - Khat B for the nc rows of B and Y (n_pad columns, stride ld).
+static int kmat_residual(const bta_gp_model* m, const double* B, const double* Y, int64_t ld,
+                         int nc, double* R, hipStream_t s) {
+  if (m->d > 32 || nc < 1) {
+    set_error("refinement: d <= 32 and at least one right-hand side needed");
+    return BTA_ERR_UNSUPPORTED;
+  }
+  const dim3 grid((unsigned)(m->n_pad / 64), (unsigned)((nc + KR_RHS - 1) / KR_RHS));
+  hipLaunchKernelGGL(kmat_residual_kernel, grid, dim3(256), 0, s, m->kind, m->train_xs, m->n,
+                     m->d, m->outputscale, m->noise, B, Y, ld, nc, R);
+  BTA_LAUNCH_CHECK("kmat_residual_kernel");
+  return BTA_OK;
+}
+
+__global__ void add_rows_kernel(double* X, const double* D, int64_t ld, int64_t cols) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < cols) X[(int64_t)blockIdx.y * ld + c] += D[(int64_t)blockIdx.y * ld + c];
+}
+
 __global__ void fill_double_kernel(double* p, int64_t n, double v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -133,8 +219,9 @@ int gp_cache_build(const bta_gp_model* m, const double* y, double* L, double* Li
   double* R1 = (double*)(base + p.take((size_t)64 * np * 8 * 3));
   double* T1 = R1 + 64 * np;
   double* A1 = T1 + 64 * np;
-  int rc = covar(m->kind, m->train_xs, m->n, m->train_xs, m->n, m->d, m->outputscale, m->noise,
-                 false, L, np, np, np, s, /*pad_identity=*/1);
+  // the factor is of Khat + chol_jitter I (Khat = s k(X, X) + noise I)
+  int rc = covar(m->kind, m->train_xs, m->n, m->train_xs, m->n, m->d, m->outputscale,
+                 m->noise + m->chol_jitter, false, L, np, np, np, s, /*pad_identity=*/1);
   if (rc) return rc;
   rc = potrf(L, np, np, Linv, np, max_tries, 1e-8, jitter_used, wf, fact_bytes, s);
   if (rc) return rc;
@@ -146,6 +233,16 @@ int gp_cache_build(const bta_gp_model* m, const double* y, double* L, double* Li
   rc = solve_alpha(Linv, np, r, R1, T1, A1, s);
   if (rc) return rc;
   BTA_HIP(hipMemcpyAsync(alpha, A1, np * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // refinement against Khat: alpha += (L L^T)^-1 (r - Khat alpha)
+  for (int it = 0; it < m->refine_steps; ++it) {
+    rc = kmat_residual(m, alpha, r, np, 1, A1, s);
+    if (rc) return rc;
+    rc = solve_alpha(Linv, np, A1, R1, T1, A1, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(add_rows_kernel, dim3((unsigned)((np + 255) / 256), 1), dim3(256), 0, s,
+                       alpha, A1, np, np);
+    BTA_LAUNCH_CHECK("add_rows_kernel");
+  }
   return BTA_OK;
 }
 
@@ -282,6 +379,26 @@ int qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st, const do
   const double v2 = m->y_std * m->y_std;
   rc = gemm(0, nbp, nbp, np, -v2, vb, np, vb, np, v2, kbb, nbp, sg, nbp, s);
   if (rc) return rc;
+  // refinement of A^T = K(X_b, X) Khat^-1 against Khat (vb is free now, and kb
+  // after the residual): A^T += (K(X_b, X) - A^T Khat) L^-T L^-1
+  for (int it = 0; it < m->refine_steps; ++it) {
+    if (it > 0) {
+      rc = covar(m->kind, st->xb_s, n_b, m->train_xs, m->n, m->d, m->outputscale, 0.0, false, kb,
+                 np, nbp, np, s);
+      if (rc) return rc;
+    }
+    rc = kmat_residual(m, at, kb, np, n_b, vb, s);
+    if (rc) return rc;
+    rc = gemm(BTA_GEMM_B_LOWER, nbp, np, np, 1.0, vb, np, m->linv, np, 0.0, nullptr, 0, kb, np,
+              s);
+    if (rc) return rc;
+    rc = gemm(BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, nbp, np, np, 1.0, kb, np, m->linv, np, 0.0,
+              nullptr, 0, vb, np, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(add_rows_kernel, dim3((unsigned)((np + 255) / 256), (unsigned)n_b),
+                       dim3(256), 0, s, at, vb, np, np);
+    BTA_LAUNCH_CHECK("add_rows_kernel");
+  }
   rc = potrf(sg, nbp, nbp, nullptr, 0, st->max_tries, st->jitter, jitter_used, base + w.fact,
              potrf_ws_bytes(nbp), s);
   if (rc) return rc;

@@ -69,10 +69,13 @@ struct H2Args {
   double alpha;
   int seg_chunks;           // 0: FP32 sums over all of K; > 0: FP64 sums of FP32
                             // segments of seg_chunks x 32 K
+  int xcd_groups;           // set by the plan: > 1 deals column strips to G worker
+                            // groups (one per XCD), see gemm_h2_body
 };
 int gemm_h2(const H2Args& a, hipStream_t s);
-// The tile width and worker count gemm_h2 would launch with (after its checks).
-int gemm_h2_plan(const H2Args& a, int* bn, int* workers);
+// The tile width, worker count and worker groups gemm_h2 would launch with
+// (after its checks).
+int gemm_h2_plan(const H2Args& a, int* bn, int* workers, int* xcd_groups = nullptr);
 int split_f16(bool f64, const void* X, int64_t ldx, int64_t rows, int64_t cols,
               int64_t rows_pad, int64_t cols_pad, float scale, bool transpose, _Float16* hi,
               _Float16* lo, int64_t ldo, hipStream_t s);

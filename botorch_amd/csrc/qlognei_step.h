@@ -1,0 +1,632 @@
+// The per-restart MC stage of the qLogNEI step and its adjoint (mc_body), the
+// per-model launch arguments of the whole step (StepArgs), shared by the
+// one-model step (qlognei.hip) and the step over many models (qbatch.hip).
+// The algorithm is described at the top of qlognei.hip.
+#pragma once
+#include "bta_common.h"
+#include "gemm.h"
+#include "gemm_h2_body.h"
+#include "kernels.h"
+
+namespace bta {
+
+constexpr int MC_MAXD = 32;
+constexpr int MC_MAXNB_SLOTS = 4;  // n_b <= 256
+// LDS bound of the MC stage, the same for the standalone and the fused
+// launches so that a state is accepted by both or by neither (the value-only
+// call runs the standalone kernel, value+grad may run the fused one).
+constexpr size_t MC_LDS_MAX = 64 * 1024;
+
+struct McArgs {
+  const void* vt;
+  int64_t ldv;
+  double* gq;         // R x q: dV = gq V per restart block (-2 s^2 gA)
+  const double* xs;
+  const double* bs;
+  const double* Lb;
+  const double* Lb_levels;  // joint mode: chol(Sigma_bb + j_t I), t = 0..max_tries, or NULL
+  int lb_levels_mask;       // bit t: Lb_levels[t] exists
+  const double* Zt;
+  const double* best_f;
+  const double* P;    // p_parts x (R x ne) partial sums of K_x [alpha | A] (FP64)
+  int64_t p_stride;   // elements between partials
+  int p_parts;
+  double* dext;       // R x ne: adjoint of P
+  double* value;
+  double* dxs_extra;
+  int* status;
+  int d, n_pad, n_b, S, kind, max_tries, grad;
+  double outputscale, y_mean, y_std, c, tau_relu, tau_max, jitter;
+};
+
+// LDS layout (doubles) shared by host sizing and the kernel.
+struct McLayout {
+  int xq, red, mu, dmu, sqq, lbr, ga, lnew, dlnew, dsqb, u, G, scal, pv, vv, hh, total;
+  __host__ __device__ McLayout(int Q, int n_b, int S) {
+    const int m = n_b + Q;
+    int o = 0;
+    xq = o; o += Q * MC_MAXD;
+    red = o; o += 4 * 64;
+    pv = o; o += Q * (n_b + 1);
+    vv = o; o += Q * (Q + 1) / 2;
+    hh = o; o += Q * m;
+    mu = o; o += Q;
+    dmu = o; o += Q;
+    sqq = o; o += Q * Q;
+    lbr = o; o += Q * Q;
+    ga = o; o += Q * Q;
+    lnew = o; o += Q * m;
+    dlnew = o; o += Q * m;
+    dsqb = o; o += Q * (n_b > 0 ? n_b : 1);
+    u = o; o += S;
+    G = o; o += S * Q;
+    scal = o; o += 8;
+    total = o;
+  }
+};
+
+// log(tau * (softplus(x/tau) + 0.1 / (1 + (x/tau)^2))) and its derivative
+// (utils/safe_math.py:298-325), evaluated in TM (FP64, or FP32 for the mixed
+// modes' sample arithmetic) from the FP64 difference x.
+template <typename TM>
+__device__ __forceinline__ void log_fatplus_grad(double x, double tau, double* v, double* dv) {
+  const TM z = (TM)(x / tau);
+  TM sp, sig;
+  if (z > (TM)20) {  // torch softplus threshold
+    sp = z;
+    sig = (TM)1;
+  } else {
+    const TM e = exp(z);
+    sp = log1p(e);
+    sig = e / ((TM)1 + e);
+  }
+  const TM z2 = (TM)1 + z * z;
+  const TM cz = (TM)1 / z2;
+  const TM f = sp + (TM)0.1 * cz;
+  const TM tt = (TM)tau;
+  *v = (double)log(tt * f);
+  // dv = (sig - 0.2 z cz^2) / (f tau), with cz / f = 1 / (sp (1 + z^2) + 0.1) so
+  // that no intermediate underflows for large |z|
+  const TM cf = sp > (TM)0 ? (TM)1 / (sp * z2 + (TM)0.1) : (TM)10;
+  *dv = (double)((sig / f - (TM)0.2 * z * cz * cf) / tt);
+}
+
+// block-wide reduction of NV values (NV <= 64); every thread gets the result
+// in out[]; uses red (4*64 doubles).  All threads must call.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red, double (&out)[NV]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (NV <= 32) {
+    int e;
+    const double s =
+        wave_sum_spread<(NV <= 32 ? NV : 1)>(*(const double(*)[NV <= 32 ? NV : 1])v, &e);
+    if (e >= 0) red[wv * 64 + e] = s;
+  } else {
+    double s[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] = v[i];
+    wave_sum_n<NV>(s);
+    if (lane == 0)
+#pragma unroll
+      for (int i = 0; i < NV; ++i) red[wv * 64 + i] = s[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) out[i] = red[i] + red[64 + i] + red[128 + i] + red[192 + i];
+  __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void block_max(double (&v)[NV], double* red, double (&out)[NV]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = wave_max(v[i]);
+    if (lane == 0) red[wv * 64 + i] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    out[i] = fmax(fmax(red[i], red[64 + i]), fmax(red[128 + i], red[192 + i]));
+  __syncthreads();
+}
+
+__device__ __forceinline__ double block_max1(double v, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double s = wave_max(v);
+  if (lane == 0) red[wv * 64] = s;
+  __syncthreads();
+  double r = fmax(fmax(red[0], red[64]), fmax(red[128], red[192]));
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double sq_dist(const double* a, const double* b, int d) {
+  double r2 = 0.0;
+  for (int j = 0; j < d; ++j) {
+    double df = a[j] - b[j];
+    r2 = fma(df, df, r2);
+  }
+  return r2;
+}
+
+// In-register Cholesky of a Q x Q SPD matrix (row-major, lower result).
+template <int Q>
+__device__ bool small_chol(const double* A, double jit, double* L) {
+  for (int i = 0; i < Q * Q; ++i) L[i] = 0.0;
+  for (int j = 0; j < Q; ++j) {
+    double s = A[j * Q + j] + jit;
+    for (int k = 0; k < j; ++k) s -= L[j * Q + k] * L[j * Q + k];
+    if (!(s > 0.0)) return false;
+    const double dj = sqrt(s), rdj = 1.0 / dj;
+    L[j * Q + j] = dj;
+    for (int i = j + 1; i < Q; ++i) {
+      double t = A[i * Q + j];
+      for (int k = 0; k < j; ++k) t -= L[i * Q + k] * L[j * Q + k];
+      L[i * Q + j] = t * rdj;
+    }
+  }
+  return true;
+}
+
+// Restart b's MC stage and its adjoint, with sm the McLayout area in LDS.
+template <int Q, typename TV>
+__device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm) {
+  const McLayout lay(Q, a.n_b, a.S);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int d = a.d, n_b = a.n_b, S = a.S, m = n_b + Q;
+  const double ys = a.y_std, ys2 = a.y_std * a.y_std;
+  double* xq = sm + lay.xq;
+  double* red = sm + lay.red;
+  double* mu = sm + lay.mu;
+  double* dmu = sm + lay.dmu;
+  double* sqq = sm + lay.sqq;
+  double* lbr = sm + lay.lbr;
+  double* gA = sm + lay.ga;
+  double* lnew = sm + lay.lnew;
+  double* dlnew = sm + lay.dlnew;
+  double* dsqb = sm + lay.dsqb;
+  double* uu = sm + lay.u;
+  double* G = sm + lay.G;
+  double* scal = sm + lay.scal;
+  const TV* vt = (const TV*)a.vt;
+  const int64_t r0 = (int64_t)b * Q;
+
+  for (int i = tid; i < Q * d; i += 256) xq[(i / d) * MC_MAXD + i % d] = a.xs[(r0 + i / d) * d + i % d];
+  if (tid == 0) scal[1] = 0.0;
+
+  // ---- P rows (fixed-order sums of their column-tile partials, one value per
+  // wave task, partials across the lanes) and V_b V_b^T over the n_pad
+  // columns of V (16-byte loads, all issued before the products) ----------
+  constexpr int NQ = Q * (Q + 1) / 2;
+  double* pv = sm + lay.pv;  // Q x (n_b + 1): P[r0 + i][j]
+  double* vv = sm + lay.vv;  // packed lower triangle of V_b V_b^T
+  {
+    // four tasks per wave per pass, their loads issued together
+    const int ne = n_b + 1;
+    for (int t0 = wv * 4; t0 < Q * ne; t0 += 16) {
+      double s[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u;
+        s[u] = 0.0;
+        if (t < Q * ne) {
+          const double* p = a.P + (r0 + t / ne) * ne + t % ne;
+          for (int pt = lane; pt < a.p_parts; pt += 64) s[u] += p[pt * a.p_stride];
+        }
+      }
+      int e;
+      const double tot = wave_sum_spread<4>(s, &e);
+      if (e >= 0 && t0 + e < Q * ne) pv[t0 + e] = tot;
+    }
+    constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
+    typedef TV vec_t __attribute__((ext_vector_type(VW)));
+    double acc[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) acc[i] = 0.0;
+#pragma unroll 2
+    for (int k = tid * VW; k < a.n_pad; k += 256 * VW) {
+      vec_t v[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) v[i] = *(const vec_t*)(vt + (r0 + i) * a.ldv + k);
+#pragma unroll
+      for (int e = 0; e < VW; ++e) {
+        int idx = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+#pragma unroll
+          for (int j = 0; j <= i; ++j) {
+            acc[idx] = fma((double)v[i][e], (double)v[j][e], acc[idx]);
+            ++idx;
+          }
+      }
+    }
+    double tot[NQ];
+    block_sum<NQ>(acc, red, tot);
+    if (tid < NQ) vv[tid] = tot[tid];
+  }
+  __syncthreads();
+  if (tid < Q * Q) {
+    const int i = tid / Q, j = tid % Q;
+    const int ii = i > j ? i : j, jj = i > j ? j : i;
+    const double kqq = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d),
+                                  a.outputscale);
+    sqq[i * Q + j] = ys2 * (kqq - vv[ii * (ii + 1) / 2 + jj]);
+  }
+  if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + pv[tid * (n_b + 1)]);
+  // ---- Sigma_qb into lnew[:, :n_b] (and a copy in dsqb for the joint mode)
+  const bool joint = a.Lb_levels != nullptr;
+  for (int e = tid; e < Q * n_b; e += 256) {
+    const int i = e / n_b, kb = e % n_b;
+    const double kqb = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d),
+                                  a.outputscale);
+    const double sqb = ys2 * (kqb - pv[i * (n_b + 1) + 1 + kb]);
+    lnew[i * m + kb] = sqb;
+    if (joint) dsqb[e] = sqb;
+  }
+  __syncthreads();
+
+  // Cached root (a.Lb): one substitution, then the q x q Schur block's
+  // Cholesky with psd_safe_cholesky's jitter escalation.  Joint mode (the
+  // reference's fallback to sampling from the joint posterior's Cholesky): the
+  // jitter j_t of try t goes on the whole (n_b + q) x (n_b + q) diagonal, so try
+  // t uses the baseline factor Lb_t = chol(Sigma_bb + j_t I) (a.Lb_levels, where
+  // bit t of a.lb_levels_mask says it exists) and redoes the substitution.
+  const double* Lb = a.Lb;
+  int st_bits = 0;
+  for (int t = 0; t <= (joint ? a.max_tries : 0); ++t) {
+    if (joint) {
+      if (!((a.lb_levels_mask >> t) & 1)) continue;  // the baseline block fails too
+      Lb = a.Lb_levels + (int64_t)t * n_b * n_b;
+      if (t > 0) {
+        for (int e = tid; e < Q * n_b; e += 256) lnew[(e / n_b) * m + e % n_b] = dsqb[e];
+        __syncthreads();
+      }
+    }
+    // ---- Lbl = Sigma_qb Lb^-T: forward substitution, one wave per row ----
+    for (int i = wv; i < Q; i += 4) {
+      double rhs[MC_MAXNB_SLOTS];
+#pragma unroll
+      for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
+        const int j = lane + 64 * t2;
+        rhs[t2] = j < n_b ? lnew[i * m + j] : 0.0;
+      }
+      for (int kb = 0; kb < n_b; ++kb) {
+        const int slot = kb >> 6, owner = kb & 63;
+        double mine = 0.0;
+#pragma unroll
+        for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2)
+          if (t2 == slot) mine = rhs[t2];
+        const double y = __shfl(mine, owner, 64) / Lb[kb * n_b + kb];
+#pragma unroll
+        for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
+          const int j = lane + 64 * t2;
+          if (j == kb) rhs[t2] = y;
+          else if (j > kb && j < n_b) rhs[t2] -= Lb[j * n_b + kb] * y;
+        }
+      }
+#pragma unroll
+      for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
+        const int j = lane + 64 * t2;
+        if (j < n_b) lnew[i * m + j] = rhs[t2];
+      }
+    }
+    __syncthreads();
+
+    // ---- Schur complement and its Cholesky --------------------------------
+    if (tid < Q * Q) {
+      const int i = tid / Q, j = tid % Q;
+      double s = 0.0;
+      for (int kb = 0; kb < n_b; ++kb) s = fma(lnew[i * m + kb], lnew[j * m + kb], s);
+      gA[i * Q + j] = sqq[i * Q + j] - s;  // gA used as scratch for the Schur block
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int st = 0;
+      bool ok = false;
+      bool has_nan = false;
+      for (int i = 0; i < Q * Q; ++i) has_nan |= (gA[i] != gA[i]);
+      const int first = joint ? t : 0, last = joint ? t : a.max_tries;
+      for (int attempt = first; attempt <= last && !has_nan; ++attempt) {
+        const double jit = attempt == 0 ? 0.0 : a.jitter * pow(10.0, (double)(attempt - 1));
+        ok = small_chol<Q>(gA, jit, lbr);
+        if (ok) {
+          if (attempt > 0) st |= BTA_STATUS_JITTER;
+          break;
+        }
+      }
+      if (has_nan) st |= BTA_STATUS_NAN;
+      else if (!ok) st |= BTA_STATUS_NOT_PSD;
+      if (!ok)
+        for (int i = 0; i < Q * Q; ++i) lbr[i] = NAN;
+      scal[0] = (double)st;
+    }
+    __syncthreads();
+    st_bits = (int)scal[0];
+    if (!joint || !(st_bits & BTA_STATUS_NOT_PSD) || (st_bits & BTA_STATUS_NAN)) break;
+    __syncthreads();  // every thread has read scal[0] before the next try rewrites it
+  }
+  if (joint && a.lb_levels_mask == 0 && tid == 0) {  // no baseline factor at any jitter
+    scal[0] = (double)BTA_STATUS_NOT_PSD;
+    for (int i = 0; i < Q * Q; ++i) lbr[i] = NAN;
+  }
+  __syncthreads();
+  for (int e = tid; e < Q * Q; e += 256) {
+    const int i = e / Q, j = e % Q;
+    lnew[i * m + n_b + j] = lbr[i * Q + j];
+  }
+  __syncthreads();
+
+  // ---- samples, log improvement, fat-max over q ---------------------------
+  // The per-sample transcendental arithmetic runs in TM: FP64 in the FP64
+  // mode, FP32 in the mixed modes (samples and best_f stay FP64).
+  typedef TV TM;
+  const double tr = a.tau_relu, tm = a.tau_max;
+  bool bad = false;
+#pragma unroll 2
+  for (int s = tid; s < S; s += 256) {
+    double y[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) y[i] = mu[i];
+    for (int k = 0; k < m; ++k) {
+      const double z = a.Zt[(int64_t)k * S + s];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) y[i] = fma(lnew[i * m + k], z, y[i]);
+    }
+    const double bf = a.best_f[s];
+    double v[Q], dv[Q];
+    double M = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      bad |= !isfinite(y[i]);
+      log_fatplus_grad<TM>(y[i] - bf, tr, &v[i], &dv[i]);
+      M = fmax(M, v[i]);
+    }
+    TM P = 0, sdp = 0, dp[Q];
+    int nties = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const TM z = (TM)((M - v[i]) / tm);
+      const TM den = (TM)2 + (TM)2 * z + z * z;
+      P += (TM)2 / den;
+      dp[i] = (TM)-2 * ((TM)2 + (TM)2 * z) / (den * den);
+      sdp += dp[i];
+      nties += (v[i] == M);
+    }
+    uu[s] = M + tm * (double)log(P);
+    const TM cm = ((TM)1 + sdp / P) / (TM)nties;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const TM du = (v[i] == M ? cm : (TM)0) - dp[i] / P;
+      G[s * Q + i] = (double)du * dv[i];
+    }
+  }
+  if (bad) scal[1] = 1.0;  // every writer stores the same value
+  __syncthreads();
+
+  // ---- logmeanexp over samples --------------------------------------------
+  double lmax = -INFINITY;
+  for (int s = tid; s < S; s += 256) lmax = fmax(lmax, uu[s]);
+  const double Mu = block_max1(lmax, red);
+  double lsum[1] = {0.0};
+  for (int s = tid; s < S; s += 256) lsum[0] += exp(uu[s] - Mu);
+  double tsum[1];
+  block_sum<1>(lsum, red, tsum);
+  if (tid == 0) {
+    a.value[b] = Mu + log(tsum[0]) - log((double)S);
+    int st = (int)scal[0];
+    if (scal[1] != 0.0) st |= BTA_STATUS_NAN;
+    a.status[b] = st;
+  }
+  if (!a.grad) return;
+
+  // ---- dmu, d[Lbl | Lbr] = (w G)^T Z with the logmeanexp weights w --------
+  constexpr int MREG = 32 / Q - 1;  // per-thread accumulators while m <= MREG
+  if (m <= MREG) {
+    constexpr int NV = Q * (MREG + 1);
+    double acc[NV];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) acc[e] = 0.0;
+    for (int s = tid; s < S; s += 256) {
+      const double w = exp(uu[s] - Mu) / tsum[0];
+      double gw[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        gw[i] = G[s * Q + i] * w;
+        acc[i * (MREG + 1) + MREG] += gw[i];
+      }
+#pragma unroll
+      for (int k = 0; k < MREG; ++k) {
+        if (k < m) {
+          const double z = a.Zt[(int64_t)k * S + s];
+#pragma unroll
+          for (int i = 0; i < Q; ++i)
+            acc[i * (MREG + 1) + k] = fma(gw[i], z, acc[i * (MREG + 1) + k]);
+        }
+      }
+    }
+    {
+      int e;
+      const double sw = wave_sum_spread<NV>(acc, &e);
+      if (e >= 0) red[wv * 64 + e] = sw;
+    }
+    __syncthreads();
+    for (int e = tid; e < Q * (m + 1); e += 256) {
+      const int i = e / (m + 1), k = e % (m + 1);
+      const int idx = i * (MREG + 1) + (k == m ? MREG : k);
+      const double tot = red[idx] + red[64 + idx] + red[128 + idx] + red[192 + idx];
+      if (k == m) dmu[i] = tot;
+      else dlnew[i * m + k] = k < n_b + i + 1 ? tot : 0.0;
+    }
+    __syncthreads();
+  } else {
+  for (int s = tid; s < S; s += 256) {
+    const double w = exp(uu[s] - Mu) / tsum[0];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) G[s * Q + i] *= w;
+  }
+  __syncthreads();
+
+  // one wave per (i, k) for wide baselines
+  for (int task = wv; task < Q * (m + 1); task += 4) {
+    const int i = task / (m + 1), k = task % (m + 1);
+    if (k < m && k >= n_b + i + 1) {  // upper part of Lbr: no dependence
+      if (lane == 0) dlnew[i * m + k] = 0.0;
+      continue;
+    }
+    double acc = 0.0;
+    if (k == m) {
+      for (int s = lane; s < S; s += 64) acc += G[s * Q + i];
+    } else {
+      const double* zr = a.Zt + (int64_t)k * S;
+      for (int s = lane; s < S; s += 64) acc = fma(G[s * Q + i], zr[s], acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      if (k == m) dmu[i] = acc;
+      else dlnew[i * m + k] = acc;
+    }
+  }
+  __syncthreads();
+  }
+
+  // ---- Cholesky backward of Lbr (torch's formula), gA symmetric ----------
+  if (tid == 0) {
+    double T[Q * Q], Mm[Q * Q], Y[Q * Q];
+    for (int i = 0; i < Q; ++i)
+      for (int j = 0; j < Q; ++j) {
+        double t = 0.0;
+        if (j <= i)
+          for (int k = 0; k < Q; ++k) t += lbr[k * Q + i] * dlnew[k * m + n_b + j];
+        T[i * Q + j] = t;  // tril(L^T dL)
+      }
+    for (int i = 0; i < Q; ++i)
+      for (int j = 0; j < Q; ++j)
+        Mm[i * Q + j] = 0.5 * (i >= j ? T[i * Q + j] : T[j * Q + i]);
+    // reciprocal diagonal once: the two solves below multiply by it
+    double rd[Q];
+    for (int i = 0; i < Q; ++i) rd[i] = 1.0 / lbr[i * Q + i];
+    // Y = L^-T Mm : solve L^T Y = Mm (upper, back substitution per column)
+    for (int j = 0; j < Q; ++j)
+      for (int i = Q - 1; i >= 0; --i) {
+        double s = Mm[i * Q + j];
+        for (int k = i + 1; k < Q; ++k) s -= lbr[k * Q + i] * Y[k * Q + j];
+        Y[i * Q + j] = s * rd[i];
+      }
+    // gA = Y L^-1 : solve gA L = Y, i.e. for each row r: x L = y
+    for (int r = 0; r < Q; ++r)
+      for (int j = Q - 1; j >= 0; --j) {
+        double s = Y[r * Q + j];
+        for (int k = j + 1; k < Q; ++k) s -= gA[r * Q + k] * lbr[k * Q + j];
+        gA[r * Q + j] = s * rd[j];
+      }
+  }
+  __syncthreads();
+  // dLbl -= 2 gA Lbl  (Schur = Sigma_qq - Lbl Lbl^T)
+  for (int e = tid; e < Q * n_b; e += 256) {
+    const int i = e / n_b, kb = e % n_b;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) s += gA[i * Q + j] * lnew[j * m + kb];
+    dlnew[i * m + kb] -= 2.0 * s;
+  }
+  __syncthreads();
+  // ---- dSigma_qb = dLbl Lb^-1 : back substitution, one wave per row -------
+  for (int i = wv; i < Q; i += 4) {
+    double rhs[MC_MAXNB_SLOTS];
+#pragma unroll
+    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+      const int j = lane + 64 * t;
+      rhs[t] = j < n_b ? dlnew[i * m + j] : 0.0;
+    }
+    for (int kb = n_b - 1; kb >= 0; --kb) {
+      const int slot = kb >> 6, owner = kb & 63;
+      double mine = 0.0;
+#pragma unroll
+      for (int t = 0; t < MC_MAXNB_SLOTS; ++t)
+        if (t == slot) mine = rhs[t];
+      const double zz = __shfl(mine, owner, 64) / Lb[kb * n_b + kb];
+#pragma unroll
+      for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+        const int j = lane + 64 * t;
+        if (j == kb) rhs[t] = zz;
+        else if (j < kb) rhs[t] -= Lb[kb * n_b + j] * zz;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+      const int j = lane + 64 * t;
+      if (j < n_b) dsqb[i * n_b + j] = rhs[t];
+    }
+  }
+  __syncthreads();
+
+  // ---- dV = gq V with gq = -2 s^2 gA: the q x q block for the gradient stage
+  for (int e = tid; e < Q * Q; e += 256) a.gq[r0 * Q + e] = -2.0 * ys2 * gA[e];
+  for (int e = tid; e < Q * (n_b + 1); e += 256) {
+    const int i = e / (n_b + 1), k = e % (n_b + 1);
+    a.dext[(r0 + i) * (n_b + 1) + k] = k == 0 ? ys * dmu[i] : -ys2 * dsqb[i * n_b + k - 1];
+  }
+  // ---- direct input gradient through K(X, X) and K(X, X_b) (scaled) ------
+  // radial factors first (one thread each), then the d coordinates
+  double* hh = sm + lay.hh;  // Q x m: h(x_i, x_i2) for i2 < Q, h(x_i, b_kb) after
+  for (int e = tid; e < Q * m; e += 256) {
+    const int i = e / m, j2 = e % m;
+    const double* xi = xq + i * MC_MAXD;
+    const double* x2 = j2 < Q ? xq + j2 * MC_MAXD : a.bs + (j2 - Q) * d;
+    hh[e] = j2 == i ? 0.0
+                    : kern_grad_factor(a.kind, sq_dist(xi, x2, d), a.outputscale, nullptr);
+  }
+  __syncthreads();
+  for (int e = tid; e < Q * d; e += 256) {
+    const int i = e / d, j = e % d;
+    const double xij = xq[i * MC_MAXD + j];
+    double acc = 0.0;
+    for (int i2 = 0; i2 < Q; ++i2)
+      acc -= 2.0 * ys2 * gA[i * Q + i2] * hh[i * m + i2] * (xij - xq[i2 * MC_MAXD + j]);
+    for (int kb = 0; kb < n_b; ++kb)
+      acc -= ys2 * dsqb[i * n_b + kb] * hh[i * m + Q + kb] * (xij - a.bs[kb * d + j]);
+    a.dxs_extra[(r0 + i) * d + j] = acc;
+  }
+}
+
+// One launch for the adjoint GEMM U = V L^-1 and the MC stage uses the first
+// McLayout bytes of the GEMM's LDS ring (64-column tiles).
+constexpr int FUSED_LDS = H2Cfg<64, false>::STAGES * H2Cfg<64, false>::STAGE;
+static_assert(MC_LDS_MAX <= (size_t)FUSED_LDS, "MC layout must fit the fused ring");
+
+// dX = (sum of the column-tile partials + direct term) / lengthscale.
+struct FinishArgs {
+  const double* Dpart;
+  int nct;
+  int64_t R;
+  int d;
+  const double* extra;
+  const double* ls;
+  double* dX;
+};
+
+// Every launch argument of one model's value(+grad) step, and its workspace
+// plan.  Built by qlognei_step_args from the caller's buffers.
+struct StepArgs {
+  int64_t b, R, R_pad, np;
+  int q, covar_mode;            // covar_mode: 0 FP64, 1 FP32, 2 FP16 planes
+  bool f32, split, grad;
+  QsCovarArgs c;
+  H2Args fwd, adj;              // split mode: V = K_x L^-T, U = V L^-1
+  int gemm_flags;               // FP64 / mixed_f32 products through gemm()
+  void* kx;
+  void* vt;
+  const void* linv;
+  McArgs mc;
+  QsGradArgs g;
+  FinishArgs fin;
+};
+
+size_t qlognei_ws_bytes(const bta_gp_model* m, const bta_qlognei_state* st, int64_t b);
+// Checks the arguments of one step and fills A (nothing is launched).
+int qlognei_step_args(const bta_gp_model* m, const bta_qlognei_state* st, const double* X,
+                      int64_t b, double* value, double* grad, int* status, void* ws,
+                      size_t ws_bytes, StepArgs& A);
+
+}  // namespace bta

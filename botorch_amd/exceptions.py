@@ -28,3 +28,11 @@ class NotPSDError(RuntimeError):
 
 class NanError(RuntimeError):
     """NaN in a factorisation input or in posterior samples (linear_operator NanError)."""
+
+
+class BotorchWarning(Warning):
+    """Base botorch_amd warning (exceptions/warnings.py:12)."""
+
+
+class NumericalWarning(RuntimeWarning):
+    """Jitter added to a factorisation (linear_operator NumericalWarning)."""

@@ -353,6 +353,12 @@ class SingleTaskGP(torch.nn.Module):
         self._num_outputs = 1
         self._cache = None
         self._cache_key = None
+        # Numerics of the prediction caches for ill-conditioned training sets:
+        # the factor is chol(K + cholesky_jitter I) and the solves K^-1 (y - c)
+        # and K^-1 k(X, X_baseline) take refine_solves refinement steps against K
+        # itself.  The defaults (0, 0) are the reference's (plain psd_safe_cholesky).
+        self.cholesky_jitter = 0.0
+        self.refine_solves = 0
         self.to(train_X.device)
 
     # -- hyperparameters -----------------------------------------------------
@@ -390,7 +396,8 @@ class SingleTaskGP(torch.nn.Module):
         hyperparameter changed (like gpytorch's prediction strategy reset)."""
         hp = self.hyperparameters()
         key = (tuple(hp["lengthscale"].tolist()), hp["outputscale"], hp["noise"],
-               hp["constant"], self.y_stats)
+               hp["constant"], self.y_stats, float(self.cholesky_jitter),
+               int(self.refine_solves))
         if self._cache is not None and self._cache_key == key:
             return self._cache
         X = self.train_inputs[0]
@@ -419,6 +426,8 @@ class SingleTaskGP(torch.nn.Module):
         m.mean_constant = hp["constant"]
         m.y_mean = y_mean
         m.y_std = y_std
+        m.chol_jitter = float(self.cholesky_jitter)
+        m.refine_steps = int(self.refine_solves)
         Lmat = torch.empty(n_pad, n_pad, dtype=torch.float64, device=dev)
         Linv = torch.empty(n_pad, n_pad, dtype=torch.float64, device=dev)
         alpha = torch.empty(n_pad, dtype=torch.float64, device=dev)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BTA_ABI_VERSION 4
+#define BTA_ABI_VERSION 5
 
 enum {
   BTA_OK = 0,
@@ -194,13 +194,20 @@ typedef struct bta_gp_model {
   double y_mean, y_std;     /* Standardize statistics */
   const double* linv;       /* n_pad x n_pad  L^-1 (lower, zero padding); set by caller */
   const double* alpha;      /* n_pad  K^-1 (y_std - c) */
+  double chol_jitter;       /* added to the diagonal for the factorisation only: L =
+                               chol(K + chol_jitter I) (0: none) */
+  int32_t refine_steps;     /* refinement steps of the solves alpha = K^-1 r and
+                               A = K^-1 k(X, X_b) against K itself (residual from
+                               kernel values evaluated on the fly; 0: none) */
 } bta_gp_model;
 
 size_t bta_gp_cache_workspace_size(int64_t n_pad);
 
-/* Builds the prediction caches: K = k(X, X) + noise I, L = chol(K) with
- * jitter, Linv = L^-1, alpha = Linv^T Linv (y - c).  y: n standardized
- * targets.  L, Linv: n_pad x n_pad; alpha: n_pad.  SYNCHRONISES (potrf info). */
+/* Builds the prediction caches: K = k(X, X) + noise I, L = chol(K +
+ * chol_jitter I) with psd_safe_cholesky's jitter on failure, Linv = L^-1,
+ * alpha = Linv^T Linv (y - c), then refine_steps times alpha += Linv^T Linv
+ * (y - c - K alpha).  y: n standardized targets.  L, Linv: n_pad x n_pad;
+ * alpha: n_pad.  SYNCHRONISES (potrf info). */
 int bta_gp_cache_build(const bta_gp_model* m, const double* y, double* L, double* Linv,
                        double* alpha, int32_t max_tries, double* jitter_used, void* ws,
                        size_t ws_bytes, bta_stream_t stream);
@@ -242,6 +249,13 @@ typedef struct bta_qlognei_state {
   int32_t max_tries;        /* psd_safe_cholesky tries for the q x q block */
   float linv_scale;         /* power of two with max|L^-1| * linv_scale <= 1 (MIXED) */
   double jitter;            /* 1e-8 (FP64 default) */
+  const double* Lb_levels;  /* NULL: cached-root sampling with Lb.  Otherwise joint
+                               sampling (the reference's fallback when the cached-root
+                               update fails, acquisition/cached_cholesky.py:143-158):
+                               (max_tries + 1) x n_b x n_b, level t = chol(Sigma_bb +
+                               j_t I) with j_0 = 0, j_t = jitter 10^(t-1); the jitter of
+                               a restart's try t goes on its whole joint diagonal */
+  int32_t lb_levels_mask;   /* bit t: level t of Lb_levels exists (its Cholesky succeeded) */
 } bta_qlognei_state;
 
 size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int32_t n_b, int32_t S);
@@ -268,6 +282,51 @@ int bta_qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st,
                            const double* X, int64_t b, double* value, double* grad,
                            int32_t* status, void* ws, size_t ws_bytes,
                            bta_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Many independent models in one step.  Stands in for running the per-step  */
+/* evaluation above (generation/gen.py:423-485 -> acquisition/logei.py:527-  */
+/* 565, monte_carlo.py:268-290) model after model, one optimize_acqf per     */
+/* SingleTaskGP: the whole batch is five launches (one per stage of the      */
+/* step, each covering every model), with no host synchronisation, so it   */
+/* can be captured in a HIP graph.                                           */
+/* ------------------------------------------------------------------------ */
+typedef struct bta_qlognei_batch_item {
+  const bta_gp_model* model;
+  const bta_qlognei_state* state; /* prepared for this model, with q and Zt set */
+  const double* X;                /* b x q x d query points (unscaled) */
+  int64_t b;                      /* restarts of this model */
+  double* value;                  /* b */
+  double* grad;                   /* b x q x d, or NULL (the same for every item) */
+  int32_t* status;                /* b, BTA_STATUS_* bits */
+} bta_qlognei_batch_item;
+
+/* Written by bta_qlognei_batch_prepare; offsets are bytes into ws. */
+typedef struct bta_qlognei_batch_plan {
+  void* ws;
+  int32_t n_items, q, kind, kd, grad;
+  int32_t covar_blocks, mc_blocks, fin_blocks, fwd_tiles, adj_tiles, fwd_workers, adj_workers;
+  size_t off_cov, off_fwd, off_adj, off_mc, off_grad, off_fin, off_cov_start, off_mc_start,
+      off_fin_start, off_fwd_tiles, off_adj_tiles;
+} bta_qlognei_batch_plan;
+
+/* Workspace of a batch: every item's bta_qlognei_workspace_size plus the launch
+ * tables (0 for an invalid item list). */
+size_t bta_qlognei_batch_workspace_size(const bta_qlognei_batch_item* items, int32_t n_items);
+
+/* Checks the items and writes the per-model launch arguments and the tile
+ * lists into ws; fills *plan (host).  Every item must use BTA_PREC_MIXED, and
+ * the items must share q (1..5), the kernel kind and the presence of the
+ * gradient; n, d, n_b, S and b may differ.  The buffers the items name (X,
+ * outputs, model and state arrays) are baked into the tables: refill X in
+ * place between steps.  SYNCHRONISES (host -> device copy of the tables). */
+int bta_qlognei_batch_prepare(const bta_qlognei_batch_item* items, int32_t n_items,
+                              void* ws, size_t ws_bytes, bta_qlognei_batch_plan* plan,
+                              bta_stream_t stream);
+
+/* One value(+grad) step of every item of the plan: each item's outputs equal
+ * those of bta_qlognei_value_grad on its own. */
+int bta_qlognei_batch_value_grad(const bta_qlognei_batch_plan* plan, bta_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Exact marginal log likelihood value + adjoint.  Replaces gpytorch          */

@@ -329,7 +329,9 @@ class OracleQLogNEI:
     (prune_baseline=False)."""
 
     def __init__(self, gp: OracleGP, X_baseline: Tensor, num_samples: int = 512,
-                 seed: int = 0, tau_relu: float = TAU_RELU, tau_max: float = TAU_MAX):
+                 seed: int = 0, tau_relu: float = TAU_RELU, tau_max: float = TAU_MAX,
+                 cache_root: bool = True):
+        self.cache_root = cache_root
         self.gp = gp
         self.X_baseline = X_baseline
         self.S = num_samples
@@ -359,6 +361,13 @@ class OracleQLogNEI:
         b, q, _ = X.shape
         X_full = torch.cat([self.X_baseline.expand(b, -1, -1), X], dim=-2)
         mean, cov = self.gp.posterior(X_full)
+        if not self.cache_root:
+            # the joint path (cached_cholesky.py:160-171 -> sampler(posterior)):
+            # mean + L z over all n_b + q points with L = psd_safe_cholesky of the
+            # joint covariance, then the last q columns
+            L = psd_safe_cholesky(cov)
+            full = torch.einsum("bij,sj->sbi", L, self.base_samples(q)) + mean.unsqueeze(0)
+            return full[..., -q:]
         bottom_rows = cov[..., -q:, :]
         return sample_cached_cholesky(mean[..., -q:], bottom_rows, self.baseline_L,
                                       self.base_samples(q))

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_metadata_without_gpu():
     lib = _lib.load()
-    assert lib.bta_abi_version() == 4
+    assert lib.bta_abi_version() == _lib.ABI_VERSION == 5
     assert b"gfx950" in lib.bta_version()
 
 

@@ -55,6 +55,10 @@ def test_prune_inferior_points_matches_oracle(case):
     if case.startswith("small"):
         w = workloads.gp_small()
         if case == "small_max_frac":
+            # a noisier, shorter-lengthscale model spreads the argmax over 12
+            # points, so ceil(0.05 * 64) = 4 takes the truncation branch (which
+            # keeps positions into the sorted unique indices, as the reference does)
+            w = dict(w, noise=0.1, lengthscale=torch.full((6,), 0.2, dtype=torch.float64))
             max_frac = 0.05
     elif case == "headline":
         w = workloads.gp_headline()

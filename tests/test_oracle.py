@@ -347,3 +347,19 @@ def test_prune_keeps_argmax_points():
     assert torch.equal(Xp2, X[:1])
     Xp3, idx3 = O.prune_inferior_points(gp, X, num_samples=256, seed=1, max_frac=0.1)
     assert len(idx3) <= 1
+
+
+def test_joint_path_equals_cached_root_path():
+    """acquisition/cached_cholesky.py:143-171 with test/acquisition/test_logei.py:
+    513-626: sampling the q new points from the joint posterior's Cholesky
+    (cache_root=False) gives the cached-root values and gradients."""
+    X, Y, ls = _data(20, 3, seed=21)
+    gp = O.OracleGP(X, Y, O.MATERN52, ls, 1.0, 1e-3)
+    Xb = X[[2, 7, 11]]
+    a = O.OracleQLogNEI(gp, Xb, num_samples=64, seed=2)
+    b = O.OracleQLogNEI(gp, Xb, num_samples=64, seed=2, cache_root=False)
+    Xq = torch.rand(4, 2, 3, generator=torch.Generator().manual_seed(8), dtype=D)
+    va, ga = a.value_and_grad(Xq)
+    vb, gb = b.value_and_grad(Xq)
+    torch.testing.assert_close(va, vb, rtol=1e-9, atol=1e-12)
+    torch.testing.assert_close(ga, gb, rtol=1e-7, atol=1e-10)
