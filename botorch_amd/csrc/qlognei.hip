@@ -35,6 +35,7 @@
 #include "qlognei_step.h"
 
 #include <cstdlib>
+#include <mutex>
 
 namespace bta {
 
@@ -432,12 +433,81 @@ size_t qlognei_ws_bytes(const bta_gp_model* m, const bta_qlognei_state* st, int6
   return QlWs(m, st, b).total;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant of the step: its workspace is `scratch`, a static
+// device array (one copy on every GPU the library runs on, held until the
+// process ends).  Every kernel of the step writes the parts of its workspace
+// it reads before reading them (the split-K zeroing of the FP64 GEMM is a
+// kernel), so nothing in it has to start at zero, and the contents a call
+// leaves behind never reach the next call's results.  Calls that overlap on
+// one GPU share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[BTA_NOALLOC_SCRATCH_BYTES];
+
+// Device address of the current GPU's copy of `scratch`: looked up once per
+// device, with that device current, and kept.  Host code passes this address,
+// never the array's name (which on the host is a stand-in, not device memory).
+static int scratch_address(void** out) {
+  static std::mutex& mu = *new std::mutex;  // never destroyed
+  static void* addr[64] = {nullptr};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= 64) {
+    set_error("qlognei_value_grad_noalloc: device id %d out of range", dev);
+    return BTA_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) {
+      set_error("qlognei_value_grad_noalloc: no device address for the scratch array");
+      return BTA_ERR_HIP;
+    }
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return BTA_OK;
+}
+
+int qlognei_value_grad_noalloc(const bta_gp_model* m, const bta_qlognei_state* st,
+                               const double* X, int64_t b, double* value, double* grad,
+                               int* status, hipStream_t s) {
+  if (!m || !st || b < 1 || b > (1 << 24) || st->q < 1 || st->q > 8) {
+    set_error("qlognei_value_grad_noalloc: invalid arguments");
+    return BTA_ERR_INVALID;
+  }
+  const size_t need = qlognei_ws_bytes(m, st, b);
+  if (need > (size_t)BTA_NOALLOC_SCRATCH_BYTES) {
+    set_error("qlognei_value_grad_noalloc: this call needs %zu bytes of scratch, above the "
+              "%zu-byte static array (use bta_qlognei_value_grad with a workspace, or fewer "
+              "restarts per call)",
+              need, (size_t)BTA_NOALLOC_SCRATCH_BYTES);
+    return BTA_ERR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return qlognei_value_grad(m, st, X, b, value, grad, status, ws,
+                            (size_t)BTA_NOALLOC_SCRATCH_BYTES, s);
+}
+
 }  // namespace bta
 
 extern "C" size_t bta_qlognei_workspace_size(const bta_gp_model* m, const bta_qlognei_state* st,
                                              int64_t b) {
   if (!m || !st || b < 1) return 0;
   return bta::qlognei_ws_bytes(m, st, b);
+}
+
+extern "C" int bta_qlognei_value_grad_noalloc(const bta_gp_model* m,
+                                              const bta_qlognei_state* st, const double* X,
+                                              int64_t b, double* value, double* grad,
+                                              int32_t* status, bta_stream_t stream) {
+  return bta::qlognei_value_grad_noalloc(m, st, X, b, value, grad, status,
+                                         (hipStream_t)stream);
 }
 
 extern "C" int bta_qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st,

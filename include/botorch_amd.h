@@ -328,6 +328,25 @@ int bta_qlognei_batch_prepare(const bta_qlognei_batch_item* items, int32_t n_ite
  * those of bta_qlognei_value_grad on its own. */
 int bta_qlognei_batch_value_grad(const bta_qlognei_batch_plan* plan, bta_stream_t stream);
 
+/* Bytes of the static device array bta_qlognei_value_grad_noalloc works in. */
+#define BTA_NOALLOC_SCRATCH_BYTES (96u << 20)
+
+/* bta_qlognei_value_grad without a workspace argument and without any device
+ * allocation: the scratch is a static device array of
+ * BTA_NOALLOC_SCRATCH_BYTES (96 MiB) in the library, one copy per GPU, held
+ * until the process ends; its address is looked up once per device on the
+ * first call there (the stream's device must be current).  Calls whose
+ * bta_qlognei_workspace_size exceeds the array fail with BTA_ERR_WORKSPACE
+ * before anything is launched (the 512-restart, q = 4, n = 2048 step needs
+ * ~59 MB in the mixed precision, ~76 MB in FP64).  Calls that overlap on one GPU share the array and race; run them
+ * on one stream.  No counterpart in the reference: an entry point for callers
+ * that cannot allocate (code under HIP graph capture, serving loops, hosts
+ * that forbid hipMalloc).  The first call for a new shape still fills the
+ * library's host-side GEMM plan cache (host memory only). */
+int bta_qlognei_value_grad_noalloc(const bta_gp_model* m, const bta_qlognei_state* st,
+                                   const double* X, int64_t b, double* value, double* grad,
+                                   int32_t* status, bta_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Exact marginal log likelihood value + adjoint.  Replaces gpytorch          */
 /* ExactMarginalLogLikelihood forward + torch autograd backward as driven by */
