@@ -137,6 +137,10 @@ SIGNATURES = {
     "bta_qlognei_value_grad_noalloc": (ctypes.c_int, [ctypes.POINTER(GpModel),
                                                       ctypes.POINTER(QLogNEIState), _p, _i64,
                                                       _p, _p, _p, _p]),
+    "bta_log_areas_forward": (ctypes.c_int, [_i32, _p, _i64, _i64, _i32, _i32, _p, _p, _i64,
+                                             _i64, _f64, _f64, _p, _p]),
+    "bta_log_areas_backward": (ctypes.c_int, [_i32, _p, _p, _i64, _i64, _i32, _i32, _p, _p,
+                                              _i64, _i64, _f64, _f64, _p, _p]),
     "bta_mll_workspace_size": (_sz, [_i64, _i32]),
     "bta_mll_value_grad": (ctypes.c_int, [_i32, _p, _p, _i64, _i32, ctypes.POINTER(_f64), _f64,
                                           _f64, _f64, _i32, ctypes.POINTER(_f64), _p, _sz, _p]),

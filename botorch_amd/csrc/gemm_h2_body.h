@@ -233,23 +233,27 @@ __device__ __forceinline__ void gemm_h2_tile(const H2Args& g, const int tm, cons
 
 // Worker w of W: the tiles h2_tile_of(jj, w, W), jj = 0, 1, ... of the
 // problem's longest-first tile order.  With g.xcd_groups = G > 1 (triangular
-// B, tiles ordered column strip by column strip), the workers form G groups by
-// w % G - the XCD a workgroup is dispatched to - and group x takes the column
-// strips x, x + G, x + 2G, ... of that order, so all row tiles of a strip of
-// B run on one XCD and read the strip through one L2; within the group the
-// tiles are dealt as above over its W / G workers (W a multiple of G).
+// B, tiles ordered column strip by column strip, NT a multiple of 2 G), the
+// workers form G groups by w % G - the XCD a workgroup is dispatched to - and
+// group x takes the strips x, 2G - 1 - x, 2G + x, 4G - 1 - x, ... of that order
+// (a snake, so that every group's strips add up to the same K length), so all
+// row tiles of a strip of B run on one XCD and read the strip through one L2;
+// within the group the tiles are dealt as above over its W / G workers (W a
+// multiple of G), still longest first.
 template <int BN, bool ACC64>
 __device__ __forceinline__ void gemm_h2_body(const H2Args& g, int w, int W, unsigned char* lds) {
   const int MT = g.M / H2_BM, NT = g.N / BN;
   const int G = g.xcd_groups;
   if (G > 1) {
     const int x = w % G, lw = w / G, Wx = W / G;
-    const int total = ((NT - x + G - 1) / G) * MT;  // tiles of strips x, x + G, ...
+    const int total = (NT / G) * MT;  // MT row tiles of each of the group's NT / G strips
     for (int jj = 0;; ++jj) {
       const int u = h2_tile_of(jj, lw, Wx);
       if (jj * Wx >= total) break;
       if (u >= total) continue;
-      const int t = ((u / MT) * G + x) * MT + u % MT;
+      const int j = u / MT;
+      const int strip = j * G + ((j & 1) ? G - 1 - x : x);
+      const int t = strip * MT + u % MT;
       int tm, tn;
       decode_tile(t, g.flags, MT, NT, tm, tn);
       gemm_h2_tile<BN, ACC64>(g, tm, tn, lds);

@@ -328,6 +328,30 @@ int bta_qlognei_batch_prepare(const bta_qlognei_batch_item* items, int32_t n_ite
  * those of bta_qlognei_value_grad on its own. */
 int bta_qlognei_batch_value_grad(const bta_qlognei_batch_plan* plan, bta_stream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* qLogEHVI / qLogNEHVI fused log-areas.  Replaces the reference's only       */
+/* native op, csrc/logei_fused.cpp:184-373 (fused_log_areas_forward /        */
+/* _backward, bound by _FusedLogAreas at acquisition/multi_objective/        */
+/* logei.py:107-170): steps 1-4 of _compute_log_qehvi (logei.py:437-479).    */
+/* ------------------------------------------------------------------------ */
+
+/* out[b][c][s] = sum_k fatmin(fatmin_j(log_fatplus(obj[b][s][j][k] - cl[bc][c][k],
+ * tau_relu), tau_max), log(min(cu[bc][c][k], clamp) - cl[bc][c][k]), tau_max) with
+ * obj B x n_sub x i x m, cell bounds B_cells x nc x m (B_cells 1 or B, bc = 0 or b),
+ * out B x nc x n_sub; clamp 1e10 (double) / 1e8 (float).  f32 != 0: every tensor is
+ * float, else double.  1 <= i <= 32, 1 <= m <= 8. */
+int bta_log_areas_forward(int32_t f32, const void* obj, int64_t B, int64_t n_sub, int32_t i,
+                          int32_t m, const void* cell_lower, const void* cell_upper,
+                          int64_t B_cells, int64_t nc, double tau_relu, double tau_max,
+                          void* out, bta_stream_t stream);
+
+/* grad_obj (B x n_sub x i x m) = d (sum grad_out * out) / d obj for grad_out
+ * (B x nc x n_sub), the other arguments as in bta_log_areas_forward. */
+int bta_log_areas_backward(int32_t f32, const void* grad_out, const void* obj, int64_t B,
+                           int64_t n_sub, int32_t i, int32_t m, const void* cell_lower,
+                           const void* cell_upper, int64_t B_cells, int64_t nc, double tau_relu,
+                           double tau_max, void* grad_obj, bta_stream_t stream);
+
 /* Bytes of the static device array bta_qlognei_value_grad_noalloc works in. */
 #define BTA_NOALLOC_SCRATCH_BYTES (96u << 20)
 

@@ -473,3 +473,29 @@ def exact_mll_adjoint(kind: int, X: Tensor, y: Tensor, lengthscale: Tensor,
         g_noise = 0.5 * torch.diagonal(W).sum()
         g_c = a.sum()
     return g_ls, g_os, g_noise, g_c
+
+
+# ----------------------------------------------------------------------------------
+# qLogEHVI log-areas (acquisition/multi_objective/logei.py:437-479, steps 1-4 of
+# _compute_log_qehvi, the computation csrc/logei_fused.cpp fuses)
+# ----------------------------------------------------------------------------------
+def fatmin(x: Tensor, dim: int, tau: float, alpha: float = 2.0) -> Tensor:
+    """safe_math.py:358-381: -fatmax(-x)."""
+    return -fatmax(-x, dim=dim, tau=tau, alpha=alpha)
+
+
+def log_areas(obj_subsets: Tensor, cell_lower: Tensor, cell_upper: Tensor,
+              tau_relu: float = TAU_RELU, tau_max: float = TAU_MAX) -> Tensor:
+    """obj_subsets: B x n_sub x i x m; cell bounds nc x m or B x nc x m.
+    Returns B x nc x n_sub: sum_k fatmin(fatmin_j(log_fatplus(obj - l_c)), log(u_c - l_c))
+    with u clamped at 1e10 (FP64) / 1e8 (FP32), as _log_improvement, _smooth_min,
+    _log_cell_lengths and the sum over objectives compute it."""
+    cl = cell_lower if cell_lower.dim() == 3 else cell_lower.unsqueeze(0)
+    cu = cell_upper if cell_upper.dim() == 3 else cell_upper.unsqueeze(0)
+    Z = obj_subsets.unsqueeze(1) - cl[:, :, None, None, :]     # B x nc x n_sub x i x m
+    li = log_fatplus(Z, tau=tau_relu)
+    lim = fatmin(li, dim=-2, tau=tau_max)                       # B x nc x n_sub x m
+    clamp = 1e10 if obj_subsets.dtype == torch.float64 else 1e8
+    lcl = (cu.clamp_max(clamp) - cl).log()[:, :, None, :]       # B' x nc x 1 x m
+    lm = fatmin(torch.stack(torch.broadcast_tensors(lim, lcl), dim=-1), dim=-1, tau=tau_max)
+    return lm.sum(dim=-1)

@@ -363,3 +363,29 @@ def test_joint_path_equals_cached_root_path():
     vb, gb = b.value_and_grad(Xq)
     torch.testing.assert_close(va, vb, rtol=1e-9, atol=1e-12)
     torch.testing.assert_close(ga, gb, rtol=1e-7, atol=1e-10)
+
+
+def test_log_areas_limit_is_the_log_box_volume():
+    """acquisition/multi_objective/logei.py:437-479: as tau_relu, tau_max -> 0 the
+    log-area of a subset in a cell tends to sum_k log(min(min_j (obj_jk - l_k),
+    u_k - l_k)) wherever every improvement is positive; a sign slip, a max for a
+    min or a dropped cell length fails this."""
+    g = torch.Generator().manual_seed(0)
+    obj = torch.rand(3, 4, 2, 3, generator=g, dtype=D)
+    cl = torch.rand(5, 3, generator=g, dtype=D) * 0.5
+    cu = cl + torch.rand(5, 3, generator=g, dtype=D)
+    a = O.log_areas(obj, cl, cu, tau_relu=1e-9, tau_max=1e-9)
+    Z = obj.unsqueeze(1) - cl[None, :, None, None, :]
+    exact = torch.minimum(Z.min(dim=-2).values, (cu - cl)[None, :, None, :]).log().sum(-1)
+    ok = Z.min(dim=-2).values.min(-1).values > 1e-3
+    assert ok.sum() > 10
+    torch.testing.assert_close(a[ok], exact[ok], rtol=0, atol=1e-12)
+    # an unbounded cell side is clamped at 1e10 (FP64)
+    cu_inf = cu.clone()
+    cu_inf[0, 0] = float("inf")
+    b = O.log_areas(obj, cl, cu_inf)
+    c = O.log_areas(obj, cl, torch.where(torch.isinf(cu_inf), torch.full_like(cu, 1e10), cu))
+    torch.testing.assert_close(b, c)
+    # batched cells (one set per batch element) equal the shared cells repeated
+    torch.testing.assert_close(O.log_areas(obj, cl.expand(3, -1, -1), cu.expand(3, -1, -1)),
+                               O.log_areas(obj, cl, cu))
