@@ -120,6 +120,10 @@ SIGNATURES = {
     "bta_gp_posterior_workspace_size": (_sz, [ctypes.POINTER(GpModel), _i64, _i32]),
     "bta_gp_posterior": (ctypes.c_int, [ctypes.POINTER(GpModel), _p, _i64, _i32, _p, _p, _p,
                                         _p, _sz, _p]),
+    "bta_gp_posterior_backward_workspace_size": (_sz, [ctypes.POINTER(GpModel), _i64, _i32]),
+    "bta_gp_posterior_backward": (ctypes.c_int, [ctypes.POINTER(GpModel), _p, _i64, _i32, _p,
+                                                 _p, _p, _p, _sz, _p]),
+    "bta_chol_small_backward": (ctypes.c_int, [_p, _p, _i32, _i64, _i64, _p, _p]),
     "bta_qlognei_prepare_workspace_size": (_sz, [ctypes.POINTER(GpModel), _i32, _i32]),
     "bta_qlognei_prepare": (ctypes.c_int, [ctypes.POINTER(GpModel),
                                            ctypes.POINTER(QLogNEIState), _p, _p, _p, _p, _p,

@@ -88,7 +88,8 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
                  posterior_transform=None, X_pending=None, constraints=None, eta=1e-3,
                  fat: bool = True, prune_baseline: bool = True, cache_root: bool | None = None,
                  tau_max: float = TAU_MAX, tau_relu: float = TAU_RELU, marginalize_dim=None,
-                 incremental: bool = True, precision: str = "fp64"):
+                 incremental: bool = True, precision: str = "fp64",
+                 max_workspace_bytes: int | None = None):
         super().__init__()
         if objective is not None or posterior_transform is not None or constraints is not None:
             raise UnsupportedError("objectives, posterior transforms and constraints are "
@@ -111,6 +112,10 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         self.tau_max = float(tau_max)
         self.tau_relu = float(tau_relu)
         self.precision = precision
+        # A batch whose step workspace (the K_x, V and U rows of all its restarts)
+        # exceeds this many bytes runs in restart chunks on one workspace: the rows
+        # are recomputed chunk by chunk instead of being held for the whole batch.
+        self.max_workspace_bytes = max_workspace_bytes
         self.prune_baseline = prune_baseline
         self.X_pending = None
         if prune_baseline:
@@ -227,6 +232,32 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
             self._levels = (levels, mask)
         return self._levels
 
+    def _chunk(self, b: int, q: int) -> int:
+        """Restarts per library call: all b, or the most whose workspace fits in
+        max_workspace_bytes."""
+        if self.max_workspace_bytes is None:
+            return b
+        L = _lib.lib()
+        self._state.q = q
+        m = ctypes.byref(self._keep["cache"]["model"])
+
+        def need(k):
+            return L.bta_qlognei_workspace_size(m, ctypes.byref(self._state), k)
+
+        if need(b) <= self.max_workspace_bytes:
+            return b
+        lo, hi = 1, b  # need(lo) fits (or nothing does), need(hi) does not
+        if need(1) > self.max_workspace_bytes:
+            raise UnsupportedError(f"max_workspace_bytes={self.max_workspace_bytes} is below "
+                                   f"one restart's workspace ({need(1)} bytes)")
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if need(mid) <= self.max_workspace_bytes:
+                lo = mid
+            else:
+                hi = mid
+        return lo
+
     def _launch(self, X: Tensor, need_grad: bool, joint: bool = False):
         """Enqueue one value(+grad) evaluation; returns device tensors
         (value b, grad b x q x d or None, status b).  joint: sample from the
@@ -238,7 +269,8 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
             raise UnsupportedError("q <= 8 is supported")
         Xc = X.to(device=self.X_baseline.device, dtype=torch.float64).contiguous()
         Zt = self.base_samples_t(q)
-        ws = self._workspace(b, q)
+        chunk = self._chunk(b, q)
+        ws = self._workspace(chunk, q)
         st = self._state
         st.q = q
         st.Zt = _lib.ptr(Zt)
@@ -250,11 +282,13 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
             st.Lb_levels = _lib.ptr(levels)
             st.lb_levels_mask = mask
         try:
-            _lib.check(L.bta_qlognei_value_grad(ctypes.byref(self._keep["cache"]["model"]),
-                                                ctypes.byref(st), _lib.ptr(Xc), b,
-                                                _lib.ptr(value), _lib.ptr(grad),
-                                                _lib.ptr(status), _lib.ptr(ws), ws.numel(),
-                                                _lib.stream()), "qlognei_value_grad")
+            for r0 in range(0, b, chunk):
+                r1 = min(b, r0 + chunk)
+                _lib.check(L.bta_qlognei_value_grad(
+                    ctypes.byref(self._keep["cache"]["model"]), ctypes.byref(st),
+                    _lib.ptr(Xc[r0:r1]), r1 - r0, _lib.ptr(value[r0:r1]),
+                    None if grad is None else _lib.ptr(grad[r0:r1]), _lib.ptr(status[r0:r1]),
+                    _lib.ptr(ws), ws.numel(), _lib.stream()), "qlognei_value_grad")
         finally:
             st.Lb_levels = None
             st.lb_levels_mask = 0

@@ -826,6 +826,76 @@ int trtri_lower(const double* L, int64_t ldl, double* X, int64_t ldx, int64_t n_
   return BTA_OK;
 }
 
+// Adjoint of the Cholesky factorisation of one n x n matrix per workgroup
+// (n <= 64): with P = Phi(L^T dL) (lower triangle, halved diagonal),
+// S = L^-T P L^-1 and dA = (S + S^T) / 2, the form torch's Cholesky backward
+// takes (the reference differentiates its posterior samples through it).  L and
+// dL are read from their lower triangles; dA is written in full.
+__global__ __launch_bounds__(256) void chol_small_backward_kernel(const double* L, const double* dL,
+                                                                  int n, int64_t stride,
+                                                                  double* dA) {
+  __shared__ double Ls[LEAF][LP];
+  __shared__ double W[LEAF][LP];
+  const int t = threadIdx.x;
+  const double* Lb = L + blockIdx.x * stride;
+  const double* dLb = dL + blockIdx.x * stride;
+  for (int e = t; e < n * n; e += 256) {
+    const int i = e / n, j = e % n;
+    Ls[i][j] = j <= i ? Lb[e] : 0.0;
+  }
+  __syncthreads();
+  // W = Phi(L^T dL): W[i][j] = sum_{k >= i} L[k][i] dL[k][j] for j <= i (halved at j == i)
+  for (int e = t; e < n * n; e += 256) {
+    const int i = e / n, j = e % n;
+    double v = 0.0;
+    if (j <= i) {
+      for (int k = i; k < n; ++k) v = fma(Ls[k][i], dLb[k * n + j], v);
+      if (j == i) v *= 0.5;
+    }
+    W[i][j] = v;
+  }
+  __syncthreads();
+  // W <- L^-T W: back substitution on each column j (thread j)
+  if (t < n) {
+    const int j = t;
+    for (int i = n - 1; i >= 0; --i) {
+      double v = W[i][j];
+      for (int k = i + 1; k < n; ++k) v -= Ls[k][i] * W[k][j];
+      W[i][j] = v / Ls[i][i];
+    }
+  }
+  __syncthreads();
+  // W <- W L^-1: for each row i (thread i) solve x L = w from the right
+  if (t < n) {
+    const int i = t;
+    for (int j = n - 1; j >= 0; --j) {
+      double v = W[i][j];
+      for (int k = j + 1; k < n; ++k) v -= W[i][k] * Ls[k][j];
+      W[i][j] = v / Ls[j][j];
+    }
+  }
+  __syncthreads();
+  double* out = dA + blockIdx.x * stride;
+  for (int e = t; e < n * n; e += 256) {
+    const int i = e / n, j = e % n;
+    out[e] = 0.5 * (W[i][j] + W[j][i]);
+  }
+}
+
+int chol_small_backward(const double* L, const double* dL, int n, int64_t stride, int64_t batch,
+                        double* dA, hipStream_t s) {
+  if (!L || !dL || !dA || n < 1 || n > LEAF || stride < (int64_t)n * n || batch < 0 ||
+      batch > (1 << 30)) {
+    set_error("chol_small_backward: need 1 <= n <= 64, stride >= n^2");
+    return BTA_ERR_INVALID;
+  }
+  if (batch == 0) return BTA_OK;
+  hipLaunchKernelGGL(chol_small_backward_kernel, dim3((unsigned)batch), dim3(256), 0, s, L, dL, n,
+                     stride, dA);
+  BTA_LAUNCH_CHECK("chol_small_backward_kernel");
+  return BTA_OK;
+}
+
 int chol_small_batched(double* A, int n, int64_t lda, int64_t stride, int64_t batch,
                        int max_tries, double jitter0, int* status, hipStream_t s) {
   if (n < 1 || n > LEAF || lda < n || batch < 0) {
@@ -872,4 +942,10 @@ extern "C" int bta_chol_small_batched(double* A, int32_t n, int64_t lda, int64_t
   }
   return bta::chol_small_batched(A, n, lda, stride, batch, max_tries, jitter0, status,
                                  (hipStream_t)stream);
+}
+
+extern "C" int bta_chol_small_backward(const double* L, const double* dL, int32_t n,
+                                       int64_t stride, int64_t batch, double* dA,
+                                       bta_stream_t stream) {
+  return bta::chol_small_backward(L, dL, n, stride, batch, dA, (hipStream_t)stream);
 }

@@ -314,6 +314,165 @@ int gp_posterior(const bta_gp_model* m, const double* X, int64_t batch, int mq, 
   return BTA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the posterior: dX from d mean and d covar.
+//   dK_x   = y_std dmean alpha^T - y_std^2 (dC + dC^T) U,   U = V L^-1 = K_x K^-1
+//   dK_xx  = y_std^2 dC  (the prior covariance among the batch's own points)
+//   dX     = through dk/dX of both, divided by the lengthscale
+// ---------------------------------------------------------------------------
+// S_b = y_std^2 (dC_b + dC_b^T), zero-padded to mq_pad x mq_pad; dm_pad = y_std
+// dmean in the padded row layout (0 where dmean is NULL).
+__global__ void post_bwd_prep_kernel(const double* dmean, const double* dcov, int64_t batch, int mq,
+                                     int mq_pad, double y_std, double* S, double* dm_pad) {
+  const int64_t bi = blockIdx.y;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < mq_pad * mq_pad;
+       e += gridDim.x * blockDim.x) {
+    const int i = e / mq_pad, j = e % mq_pad;
+    double v = 0.0;
+    if (dcov && i < mq && j < mq)
+      v = y_std * y_std *
+          (dcov[(bi * mq + i) * mq + j] + dcov[(bi * mq + j) * mq + i]);
+    S[bi * mq_pad * mq_pad + e] = v;
+    if (j == 0) dm_pad[bi * mq_pad + i] = (dmean && i < mq) ? y_std * dmean[bi * mq + i] : 0.0;
+  }
+}
+
+// G[r][k] += dm_pad[r] alpha[k]  (the mean's part of dK_x)
+__global__ void add_outer_kernel(double* G, int64_t ldg, const double* dm_pad,
+                                 const double* alpha, int64_t n) {
+  const int64_t r = blockIdx.y;
+  const double a = dm_pad[r];
+  if (a == 0.0) return;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    G[r * ldg + k] += a * alpha[k];
+}
+
+// extra[r][j] = sum_i2 S_b[i][i2] dk(xs_i, xs_i2)/dxs_i[j] = -sum_i2 S_b[i][i2] h (xs_i - xs_i2)_j
+// (scaled coordinates; the row's own term has no gradient)
+__global__ void post_direct_grad_kernel(int kind, const double* xs, int64_t batch, int mq,
+                                        int mq_pad, int d, double s, const double* S,
+                                        double* extra) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= batch * mq_pad * d) return;
+  const int j = (int)(e % d);
+  const int64_t row = e / d;
+  const int64_t bi = row / mq_pad;
+  const int i = (int)(row % mq_pad);
+  double acc = 0.0;
+  if (i < mq) {
+    const double* xi = xs + row * d;
+    for (int i2 = 0; i2 < mq; ++i2) {
+      if (i2 == i) continue;
+      const double sv = S[(bi * mq_pad + i) * mq_pad + i2];
+      if (sv == 0.0) continue;
+      const double* x2 = xs + (bi * mq_pad + i2) * d;
+      double r2 = 0.0;
+      for (int t = 0; t < d; ++t) {
+        const double df = xi[t] - x2[t];
+        r2 = fma(df, df, r2);
+      }
+      acc -= sv * kern_grad_factor(kind, r2, s, nullptr) * (xi[j] - x2[j]);
+    }
+  }
+  extra[e] = acc;
+}
+
+// dX[(bi mq + r) d + j] = dXp[(bi mq_pad + r) d + j]
+__global__ void unpad_rows_kernel(const double* dXp, int64_t batch, int mq, int mq_pad, int d,
+                                  double* dX) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= batch * mq * d) return;
+  const int j = (int)(e % d);
+  const int64_t row = e / d;
+  const int64_t bi = row / mq;
+  const int r = (int)(row % mq);
+  dX[e] = dXp[(bi * mq_pad + r) * d + j];
+}
+
+struct PostBwdWs {
+  int mq_pad;
+  int64_t R_pad;
+  size_t xs, kx, vt, S, dm, extra, dxp, total;
+  PostBwdWs(const bta_gp_model* m, int64_t batch, int mq) {
+    mq_pad = (int)round_up(mq, 64);
+    R_pad = batch * mq_pad;
+    WsPlan p;
+    xs = p.take((size_t)R_pad * m->d * 8);
+    kx = p.take((size_t)R_pad * m->n_pad * 8);
+    vt = p.take((size_t)R_pad * m->n_pad * 8);
+    S = p.take((size_t)batch * mq_pad * mq_pad * 8);
+    dm = p.take((size_t)R_pad * 8);
+    extra = p.take((size_t)R_pad * m->d * 8);
+    dxp = p.take((size_t)R_pad * m->d * 8);
+    total = p.off;
+  }
+};
+
+int gp_posterior_backward(const bta_gp_model* m, const double* X, int64_t batch, int mq,
+                          const double* dmean, const double* dcov, double* dX, void* ws,
+                          size_t ws_bytes, hipStream_t s) {
+  if (!m || !X || !dX || batch < 1 || mq < 1 || !m->linv || !m->alpha || batch > 65535 ||
+      (!dmean && !dcov)) {
+    set_error("gp_posterior_backward: invalid arguments");
+    return BTA_ERR_INVALID;
+  }
+  const PostBwdWs w(m, batch, mq);
+  if (!ws || ws_bytes < w.total) {
+    set_error("gp_posterior_backward: workspace too small");
+    return BTA_ERR_WORKSPACE;
+  }
+  char* base = (char*)ws;
+  double* xs = (double*)(base + w.xs);
+  double* kx = (double*)(base + w.kx);
+  double* vt = (double*)(base + w.vt);
+  double* S = (double*)(base + w.S);
+  double* dm = (double*)(base + w.dm);
+  double* extra = (double*)(base + w.extra);
+  double* dxp = (double*)(base + w.dxp);
+  const int64_t np = m->n_pad, mp = w.mq_pad;
+  const int64_t tot = w.R_pad * m->d;
+  hipLaunchKernelGGL(scale_batched_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                     X, batch, mq, w.mq_pad, m->d, m->center, m->lengthscale, xs);
+  BTA_LAUNCH_CHECK("scale_batched_kernel");
+  int rc = covar(m->kind, xs, w.R_pad, m->train_xs, m->n, m->d, m->outputscale, 0.0, false, kx,
+                 np, w.R_pad, np, s);
+  if (rc) return rc;
+  // V = K_x L^-T into vt, then U = V L^-1 into kx
+  rc = gemm(BTA_GEMM_B_LOWER, w.R_pad, np, np, 1.0, kx, np, m->linv, np, 0.0, nullptr, 0, vt, np,
+            s);
+  if (rc) return rc;
+  rc = gemm(BTA_GEMM_B_KN | BTA_GEMM_B_UPPER, w.R_pad, np, np, 1.0, vt, np, m->linv, np, 0.0,
+            nullptr, 0, kx, np, s);
+  if (rc) return rc;
+  {
+    const dim3 g((unsigned)std::min<int64_t>(64, (mp * mp + 255) / 256), (unsigned)batch);
+    hipLaunchKernelGGL(post_bwd_prep_kernel, g, dim3(256), 0, s, dmean, dcov, batch, mq,
+                       w.mq_pad, m->y_std, S, dm);
+    BTA_LAUNCH_CHECK("post_bwd_prep_kernel");
+  }
+  // G = -S U per batch block (into vt), then + y_std dmean alpha^T
+  rc = gemm(BTA_GEMM_B_KN, mp, np, mp, -1.0, S, mp, kx, np, 0.0, nullptr, 0, vt, np, s,
+            (int)batch, mp * mp, mp * np, 0, mp * np);
+  if (rc) return rc;
+  if (dmean) {
+    const dim3 g((unsigned)std::min<int64_t>(64, (np + 255) / 256), (unsigned)w.R_pad);
+    hipLaunchKernelGGL(add_outer_kernel, g, dim3(256), 0, s, vt, np, dm, m->alpha, m->n);
+    BTA_LAUNCH_CHECK("add_outer_kernel");
+  }
+  hipLaunchKernelGGL(post_direct_grad_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     s, m->kind, xs, batch, mq, w.mq_pad, m->d, m->outputscale, S, extra);
+  BTA_LAUNCH_CHECK("post_direct_grad_kernel");
+  rc = covar_grad(m->kind, xs, w.R_pad, m->train_xs, m->n, m->d, m->outputscale, vt, false, np,
+                  extra, m->lengthscale, dxp, s);
+  if (rc) return rc;
+  const int64_t to = batch * mq * m->d;
+  hipLaunchKernelGGL(unpad_rows_kernel, dim3((unsigned)((to + 255) / 256)), dim3(256), 0, s, dxp,
+                     batch, mq, w.mq_pad, m->d, dX);
+  BTA_LAUNCH_CHECK("unpad_rows_kernel");
+  return BTA_OK;
+}
+
 struct PrepWs {
   int64_t nbp;
   size_t kb, vb, at, kbb, sg, lbp, arow, ka, mu, fact, mx, total;
@@ -474,6 +633,20 @@ extern "C" int bta_gp_posterior(const bta_gp_model* m, const double* X, int64_t 
                                 size_t ws_bytes, bta_stream_t stream) {
   return bta::gp_posterior(m, X, batch, mq, mean, covar, var, ws, ws_bytes,
                            (hipStream_t)stream);
+}
+
+extern "C" size_t bta_gp_posterior_backward_workspace_size(const bta_gp_model* m,
+                                                          int64_t batch, int32_t mq) {
+  if (!m || batch < 1 || mq < 1) return 0;
+  return bta::PostBwdWs(m, batch, mq).total;
+}
+
+extern "C" int bta_gp_posterior_backward(const bta_gp_model* m, const double* X, int64_t batch,
+                                         int32_t mq, const double* dmean, const double* dcov,
+                                         double* dX, void* ws, size_t ws_bytes,
+                                         bta_stream_t stream) {
+  return bta::gp_posterior_backward(m, X, batch, mq, dmean, dcov, dX, ws, ws_bytes,
+                                    (hipStream_t)stream);
 }
 
 extern "C" size_t bta_qlognei_prepare_workspace_size(const bta_gp_model* m, int32_t n_b,

@@ -451,26 +451,18 @@ class SingleTaskGP(torch.nn.Module):
             raise UnsupportedError("posterior_transform is not supported")
         if output_indices not in (None, [0]):
             raise UnsupportedError("single-output model")
-        if torch.is_grad_enabled() and X.requires_grad:
-            raise UnsupportedError(
-                "gradients through SingleTaskGP.posterior are provided by the fused "
-                "qLogNoisyExpectedImprovement path; call posterior() under no_grad")
         self.eval()
-        c = self.prediction_cache()
         X64 = X.to(torch.float64)
         batch_shape = X64.shape[:-2]
         mq, d = X64.shape[-2], X64.shape[-1]
-        Xf = X64.reshape(-1, mq, d).contiguous()
-        nb = Xf.shape[0]
-        L = _lib.lib()
+        Xf = X64.reshape(-1, mq, d)
         dev = Xf.device
-        mean = torch.empty(nb, mq, dtype=torch.float64, device=dev)
-        cov = torch.empty(nb, mq, mq, dtype=torch.float64, device=dev)
-        ws = _lib.workspace(L.bta_gp_posterior_workspace_size(ctypes.byref(c["model"]), nb, mq),
-                            dev)
-        _lib.check(L.bta_gp_posterior(ctypes.byref(c["model"]), _lib.ptr(Xf), nb, mq,
-                                      _lib.ptr(mean), _lib.ptr(cov), None, _lib.ptr(ws),
-                                      ws.numel(), _lib.stream()), "gp_posterior")
+        if torch.is_grad_enabled() and Xf.requires_grad:
+            # differentiable in X like the reference's posterior: the adjoint runs
+            # in bta_gp_posterior_backward
+            mean, cov = _PosteriorFn.apply(Xf, self)
+        else:
+            mean, cov = self._posterior_moments(Xf.detach().contiguous())
         if observation_noise:
             _, y_std = self.y_stats
             cov = cov + (y_std**2) * self.likelihood.noise.detach() * torch.eye(
@@ -479,8 +471,61 @@ class SingleTaskGP(torch.nn.Module):
         cov = cov.reshape(*batch_shape, mq, mq)
         return GPyTorchPosterior(mean, cov)
 
+    def _posterior_moments(self, Xf: Tensor):
+        """Mean (nb x mq) and covariance (nb x mq x mq), original outcome scale."""
+        c = self.prediction_cache()
+        L = _lib.lib()
+        nb, mq, _ = Xf.shape
+        dev = Xf.device
+        mean = torch.empty(nb, mq, dtype=torch.float64, device=dev)
+        cov = torch.empty(nb, mq, mq, dtype=torch.float64, device=dev)
+        ws = _lib.workspace(L.bta_gp_posterior_workspace_size(ctypes.byref(c["model"]), nb, mq),
+                            dev)
+        _lib.check(L.bta_gp_posterior(ctypes.byref(c["model"]), _lib.ptr(Xf), nb, mq,
+                                      _lib.ptr(mean), _lib.ptr(cov), None, _lib.ptr(ws),
+                                      ws.numel(), _lib.stream()), "gp_posterior")
+        return mean, cov
+
+    def _posterior_backward(self, Xf: Tensor, dmean: Tensor | None, dcov: Tensor | None):
+        """d X (nb x mq x d) of <dmean, mean> + <dcov, covariance>."""
+        c = self.prediction_cache()
+        L = _lib.lib()
+        nb, mq, d = Xf.shape
+        dev = Xf.device
+        dX = torch.empty(nb, mq, d, dtype=torch.float64, device=dev)
+        if dmean is None and dcov is None:
+            return dX.zero_()
+        ws = _lib.workspace(L.bta_gp_posterior_backward_workspace_size(
+            ctypes.byref(c["model"]), nb, mq), dev)
+        dm = None if dmean is None else dmean.to(torch.float64).contiguous()
+        dc = None if dcov is None else dcov.to(torch.float64).contiguous()
+        _lib.check(L.bta_gp_posterior_backward(ctypes.byref(c["model"]), _lib.ptr(Xf), nb, mq,
+                                               _lib.ptr(dm), _lib.ptr(dc), _lib.ptr(dX),
+                                               _lib.ptr(ws), ws.numel(), _lib.stream()),
+                   "gp_posterior_backward")
+        return dX
+
     def forward(self, X: Tensor):
         raise UnsupportedError("use posterior(); the prior MVN is not materialised")
+
+
+class _PosteriorFn(torch.autograd.Function):
+    """Posterior moments with the adjoint of bta_gp_posterior_backward (the
+    reference keeps SingleTaskGP.posterior differentiable in X,
+    models/gpytorch.py:544-610)."""
+
+    @staticmethod
+    def forward(ctx, Xf: Tensor, model: "SingleTaskGP"):
+        Xc = Xf.detach().contiguous()
+        mean, cov = model._posterior_moments(Xc)
+        ctx.model = model
+        ctx.save_for_backward(Xc)
+        return mean, cov
+
+    @staticmethod
+    def backward(ctx, dmean: Tensor | None, dcov: Tensor | None):
+        (Xc,) = ctx.saved_tensors
+        return ctx.model._posterior_backward(Xc, dmean, dcov), None
 
 
 # ----------------------------------------------------------------------------
@@ -570,7 +615,10 @@ class GPyTorchPosterior:
     @property
     def scale_tril(self) -> Tensor:
         if self._scale_tril is None:
-            self._scale_tril = psd_safe_cholesky(self._covar)
+            if torch.is_grad_enabled() and self._covar.requires_grad:
+                self._scale_tril = _CholFn.apply(self._covar)
+            else:
+                self._scale_tril = psd_safe_cholesky(self._covar)
         return self._scale_tril
 
     def rsample_from_base_samples(self, sample_shape: torch.Size, base_samples: Tensor) -> Tensor:
@@ -583,7 +631,10 @@ class GPyTorchPosterior:
         S = int(torch.Size(sample_shape).numel())
         nb = int(self._mean[..., 0].numel())
         zf = z.reshape(S, nb, m).permute(1, 0, 2).contiguous()  # nb x S x m
-        out = _batched_lower_matmul(Lt.reshape(nb, m, m), zf)  # nb x S x m: z L^T
+        if torch.is_grad_enabled() and Lt.requires_grad:
+            out = _LowerMatmulFn.apply(Lt.reshape(nb, m, m), zf)
+        else:
+            out = _batched_lower_matmul(Lt.reshape(nb, m, m), zf)  # nb x S x m: z L^T
         out = out + self._mean.reshape(nb, 1, m)
         return out.permute(1, 0, 2).reshape(*sample_shape, *self._mean.shape, 1)
 
@@ -607,3 +658,55 @@ def _batched_lower_matmul(Lt: Tensor, Z: Tensor) -> Tensor:
                             mp, 0.0, None, 0, _lib.ptr(Cp), mp, nb, Sp * mp, mp * mp, 0,
                             Sp * mp, _lib.stream()), "gemm")
     return Cp[:, :S, :m]
+
+
+class _CholFn(torch.autograd.Function):
+    """psd_safe_cholesky with the Cholesky adjoint on the GPU
+    (bta_chol_small_backward; k <= 64)."""
+
+    @staticmethod
+    def forward(ctx, A: Tensor):
+        Lt = psd_safe_cholesky(A.detach())
+        ctx.save_for_backward(Lt)
+        return Lt
+
+    @staticmethod
+    def backward(ctx, dL: Tensor):
+        (Lt,) = ctx.saved_tensors
+        k = Lt.shape[-1]
+        if k > 64:
+            raise UnsupportedError("the Cholesky adjoint is implemented for k <= 64")
+        Lf = Lt.reshape(-1, k, k).contiguous()
+        dLf = dL.to(torch.float64).reshape(-1, k, k).contiguous()
+        dA = torch.empty_like(Lf)
+        _lib.check(_lib.lib().bta_chol_small_backward(_lib.ptr(Lf), _lib.ptr(dLf), k, k * k,
+                                                      Lf.shape[0], _lib.ptr(dA), _lib.stream()),
+                   "chol_small_backward")
+        return dA.reshape(Lt.shape)
+
+
+class _LowerMatmulFn(torch.autograd.Function):
+    """Z L^T for lower L (nb x m x m) and Z (nb x S x m) on the MFMA GEMM, with
+    dL = tril(dOut^T Z) on the same GEMM."""
+
+    @staticmethod
+    def forward(ctx, Lt: Tensor, Z: Tensor):
+        ctx.save_for_backward(Z)
+        return _batched_lower_matmul(Lt.detach(), Z)
+
+    @staticmethod
+    def backward(ctx, dOut: Tensor):
+        (Z,) = ctx.saved_tensors
+        lib = _lib.lib()
+        nb, S, m = Z.shape
+        Sp, mp = _lib.round_up(S, 64), _lib.round_up(m, 64)
+        Gp = torch.zeros(nb, Sp, mp, dtype=torch.float64, device=Z.device)
+        Gp[:, :S, :m] = dOut
+        Zp = torch.zeros(nb, Sp, mp, dtype=torch.float64, device=Z.device)
+        Zp[:, :S, :m] = Z
+        Cp = torch.empty(nb, mp, mp, dtype=torch.float64, device=Z.device)
+        # C[i][j] = sum_s dOut[s][i] Z[s][j]  (A stored K x M, B stored K x N)
+        _lib.check(lib.bta_gemm(_lib.GEMM_A_KM | _lib.GEMM_B_KN, mp, mp, Sp, 1.0, _lib.ptr(Gp),
+                                mp, _lib.ptr(Zp), mp, 0.0, None, 0, _lib.ptr(Cp), mp, nb,
+                                Sp * mp, Sp * mp, 0, mp * mp, _lib.stream()), "gemm")
+        return torch.tril(Cp[:, :m, :m]), None

@@ -168,6 +168,13 @@ int bta_potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv
 int bta_trtri_lower(const double* L, int64_t ldl, double* Linv, int64_t ldinv,
                     int64_t n_pad, void* ws, size_t ws_bytes, bta_stream_t stream);
 
+/* Adjoint of the Cholesky factorisation of batch small n x n matrices (n <= 64,
+ * matrix b at L + b stride): dA = (S + S^T) / 2 with S = L^-T Phi(L^T dL) L^-1,
+ * Phi = lower triangle with the diagonal halved (torch's Cholesky backward, which
+ * the reference's posterior samples go through: posteriors/gpytorch.py:86-127). */
+int bta_chol_small_backward(const double* L, const double* dL, int32_t n, int64_t stride,
+                            int64_t batch, double* dA, bta_stream_t stream);
+
 /* Batched Cholesky of small n x n matrices (n <= 64) with per-matrix jitter
  * escalation; status[b] gets BTA_STATUS_* bits.  Strict upper triangle zeroed. */
 int bta_chol_small_batched(double* A, int32_t n, int64_t lda, int64_t stride,
@@ -221,6 +228,19 @@ size_t bta_gp_posterior_workspace_size(const bta_gp_model* m, int64_t batch, int
 int bta_gp_posterior(const bta_gp_model* m, const double* X, int64_t batch, int32_t mq,
                      double* mean, double* covar, double* var, void* ws, size_t ws_bytes,
                      bta_stream_t stream);
+
+size_t bta_gp_posterior_backward_workspace_size(const bta_gp_model* m, int64_t batch,
+                                                int32_t mq);
+
+/* Adjoint of bta_gp_posterior: dX (batch x mq x d) = d <dmean, mean> + <dcov,
+ * covar> / dX for dmean (batch x mq) and dcov (batch x mq x mq), either of which
+ * may be NULL.  With U = K*x K^-1: dK*x = y_std dmean alpha^T - y_std^2 (dcov +
+ * dcov^T) U and dK** = y_std^2 dcov, taken through dk/dX.  Stands in for the
+ * autograd backward through SingleTaskGP.posterior (models/gpytorch.py:544-610
+ * -> gpytorch ExactGP prediction), which the reference keeps differentiable. */
+int bta_gp_posterior_backward(const bta_gp_model* m, const double* X, int64_t batch, int32_t mq,
+                              const double* dmean, const double* dcov, double* dX, void* ws,
+                              size_t ws_bytes, bta_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* qLogNoisyExpectedImprovement (cache_root=True, fat=True, incremental,     */

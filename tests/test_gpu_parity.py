@@ -132,3 +132,27 @@ def test_value_only_step_matches_oracle(case, precision):
     assert torch.equal(v_only, v_full)
     with torch.no_grad():  # forward() without grad takes the same path
         assert torch.equal(acq(Xd), v_only)
+
+
+# ------------------------------------------------------------------ restart chunks
+def test_step_in_restart_chunks_equals_whole_batch():
+    """With max_workspace_bytes below the whole batch's workspace, the step runs
+    in restart chunks on one workspace (the K_x, V, U rows recomputed chunk by
+    chunk instead of held for all restarts): values, gradients and status words
+    equal the one-call step's bit for bit."""
+    w = workloads.gp_headline(b=512)
+    _say("headline step, 512 restarts, in chunks vs whole (mixed)")
+    gp = _gp(w)
+    orc = _oracle(w)
+    Xb, _ = O.prune_inferior_points(orc, w["X"], num_samples=2048, seed=123)
+    kw = dict(sampler=B.SobolQMCNormalSampler(torch.Size([w["S"]]), seed=0),
+              prune_baseline=False, precision="mixed")
+    whole = B.qLogNoisyExpectedImprovement(gp, Xb.to(DEV), **kw)
+    X = w["Xq"].to(DEV).contiguous()
+    v1, g1, s1 = whole._launch(X, True)
+    budget = whole._workspace(512, 4).numel() // 3
+    chunked = B.qLogNoisyExpectedImprovement(gp, Xb.to(DEV), max_workspace_bytes=budget, **kw)
+    assert chunked._chunk(512, 4) < 512 // 2
+    v2, g2, s2 = chunked._launch(X, True)
+    torch.cuda.synchronize()
+    assert torch.equal(v1, v2) and torch.equal(g1, g2) and torch.equal(s1, s2)
