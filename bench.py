@@ -298,6 +298,9 @@ def main():
     acq_m, w, Xb = build_headline("mixed", n, d, b, q, S)
     X = w["Xq"].to("cuda")
     n_b = Xb.shape[0]
+    # the measured tuner picks the split products' plans for this shape first
+    plans = acq_m.autotune(X)
+    log(f"autotuned plans {plans}")
 
     def step_mixed():
         v, g, st = acq_m._launch(X, True)
@@ -439,6 +442,7 @@ def main():
         "step_tflops_mixed": step_flops / t_mixed / 1e12,
         "accuracy_max_rel_err_vs_fp64_oracle": acc,
         "roofline": roof, "cpu_baseline": cpu, "mll": mll, "batched_uneven": batched,
+        "autotuned_plans": plans,
     }
     print(json.dumps(res))
 

@@ -131,6 +131,9 @@ SIGNATURES = {
                                            _p, _sz, _p]),
     "bta_qlognei_workspace_size": (_sz, [ctypes.POINTER(GpModel), ctypes.POINTER(QLogNEIState),
                                          _i64]),
+    "bta_qlognei_autotune": (ctypes.c_int, [ctypes.POINTER(GpModel),
+                                            ctypes.POINTER(QLogNEIState), _p, _i64, _p, _p, _p,
+                                            _p, _sz, ctypes.POINTER(_i32), _p]),
     "bta_qlognei_value_grad": (ctypes.c_int, [ctypes.POINTER(GpModel),
                                               ctypes.POINTER(QLogNEIState), _p, _i64, _p, _p,
                                               _p, _p, _sz, _p]),

@@ -309,6 +309,30 @@ class qLogNoisyExpectedImprovement(torch.nn.Module):
         if jit:
             warnings.warn("A not p.d., added jitter to the diagonal", RuntimeWarning)
 
+    def autotune(self, X: Tensor) -> dict:
+        """Times the plan candidates of the step's two split-FP16 products on X
+        (b x q x d) and keeps the fastest for this shape (bta_qlognei_autotune);
+        returns the chosen (tile width, workers, XCD groups) of both.  Results do
+        not change: every plan computes each output element in the same order."""
+        L = _lib.lib()
+        b, q, d = X.shape
+        Xc = X.to(device=self.X_baseline.device, dtype=torch.float64).contiguous()
+        st = self._state
+        st.q = q
+        st.Zt = _lib.ptr(self.base_samples_t(q))
+        ws = self._workspace(b, q)
+        value = torch.empty(b, dtype=torch.float64, device=Xc.device)
+        grad = torch.empty(b, q, d, dtype=torch.float64, device=Xc.device)
+        status = torch.empty(b, dtype=torch.int32, device=Xc.device)
+        chosen = (ctypes.c_int32 * 6)()
+        _lib.check(L.bta_qlognei_autotune(ctypes.byref(self._keep["cache"]["model"]),
+                                          ctypes.byref(st), _lib.ptr(Xc), b, _lib.ptr(value),
+                                          _lib.ptr(grad), _lib.ptr(status), _lib.ptr(ws),
+                                          ws.numel(), chosen, _lib.stream()), "qlognei_autotune")
+        c = list(chosen)
+        return {"forward": {"bn": c[0], "workers": c[1], "xcd_groups": c[2]},
+                "adjoint": {"bn": c[3], "workers": c[4], "xcd_groups": c[5]}}
+
     @staticmethod
     def _cached_root_failed(status: Tensor) -> bool:
         return bool(((status & (_lib.STATUS_NAN | _lib.STATUS_NOT_PSD)) != 0).any())

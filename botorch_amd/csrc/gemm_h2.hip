@@ -83,22 +83,38 @@ struct H2Plan {
 // BTA_H2_XCD=1 is set: it lets one L2 serve each L^-1 strip, but on the
 // 2048 x 2048 x 2048 step products it measured no faster (50.7-52.6 us against
 // 50.2-52.9 us per forward product, step 0.189-0.193 ms against 0.185-0.187 ms).
-static int h2_xcd_groups(const H2Args& g, int BN, int W) {
-  static const int on = [] {
-    const char* e = getenv("BTA_H2_XCD");
-    return e && e[0] == '1';
-  }();
+static int h2_xcd_allowed(const H2Args& g, int BN, int W) {
   const int G = num_cus() / 32;
-  if (!on || G < 2 || !(g.flags & (BTA_GEMM_B_LOWER | BTA_GEMM_B_UPPER)) || W % G ||
+  if (G < 2 || !(g.flags & (BTA_GEMM_B_LOWER | BTA_GEMM_B_UPPER)) || W % G ||
       (g.N / BN) % (2 * G))
     return 1;
   return G;
 }
 
-static H2Plan h2_plan(const H2Args& g, bool acc64) {
-  // never destroyed (no exit-time destructor; see plan_gemm)
+static int h2_xcd_groups(const H2Args& g, int BN, int W) {
+  static const int on = [] {
+    const char* e = getenv("BTA_H2_XCD");
+    return e && e[0] == '1';
+  }();
+  return on ? h2_xcd_allowed(g, BN, W) : 1;
+}
+
+// The plans per problem signature: filled by the cost model below on a shape's
+// first call, or by the measured tuner (gemm_h2_set_plan).  Never destroyed (no
+// exit-time destructor; see plan_gemm).
+typedef std::tuple<int, int, int, int, int> H2Key;
+static std::mutex& h2_mu() {
   static std::mutex& mu = *new std::mutex;
-  static auto& cache = *new std::map<std::tuple<int, int, int, int, int>, H2Plan>;
+  return mu;
+}
+static std::map<H2Key, H2Plan>& h2_cache() {
+  static auto& cache = *new std::map<H2Key, H2Plan>;
+  return cache;
+}
+
+static H2Plan h2_plan(const H2Args& g, bool acc64) {
+  std::mutex& mu = h2_mu();
+  auto& cache = h2_cache();
   const auto key = std::make_tuple(g.M, g.N, g.K, g.flags, (int)acc64);
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -137,6 +153,41 @@ static int h2_check(const H2Args& a) {
     return BTA_ERR_INVALID;
   }
   return BTA_OK;
+}
+
+int gemm_h2_candidates(const H2Args& g, int* out, int max) {
+  if (h2_check(g)) return 0;
+  const int ncu = num_cus();
+  const bool acc64 = g.seg_chunks > 0;
+  const int slots = acc64 ? ncu : 2 * ncu;
+  int k = 0;
+  for (int bn : {64, 128}) {
+    if (g.N % bn) continue;
+    const int total = (g.M / H2_BM) * (g.N / bn);
+    int last = -1;
+    for (int c : {slots, 3 * slots / 4, slots / 2, 3 * slots / 8, slots / 4}) {
+      const int W = std::max(1, std::min(c, total));
+      if (W == last) continue;
+      last = W;
+      for (int G : {1, h2_xcd_allowed(g, bn, W)}) {
+        if (G == 1 && k > 0 && out[3 * (k - 1)] == bn && out[3 * (k - 1) + 1] == W &&
+            out[3 * (k - 1) + 2] == 1)
+          continue;
+        if (k >= max) return k;
+        out[3 * k] = bn;
+        out[3 * k + 1] = W;
+        out[3 * k + 2] = G;
+        ++k;
+      }
+    }
+  }
+  return k;
+}
+
+void gemm_h2_set_plan(const H2Args& g, int bn, int workers, int xcd_groups) {
+  const auto key = std::make_tuple(g.M, g.N, g.K, g.flags, (int)(g.seg_chunks > 0));
+  std::lock_guard<std::mutex> lk(h2_mu());
+  h2_cache()[key] = H2Plan{bn, workers, xcd_groups};
 }
 
 int gemm_h2_plan(const H2Args& a, int* bn, int* workers, int* xcd_groups) {

@@ -76,6 +76,11 @@ int gemm_h2(const H2Args& a, hipStream_t s);
 // The tile width, worker count and worker groups gemm_h2 would launch with
 // (after its checks).
 int gemm_h2_plan(const H2Args& a, int* bn, int* workers, int* xcd_groups = nullptr);
+// The plans the measured tuner tries for a problem: up to max (bn, workers,
+// xcd_groups) triples into out; returns their number.
+int gemm_h2_candidates(const H2Args& a, int* out, int max);
+// Makes (bn, workers, xcd_groups) the plan of the problem's signature.
+void gemm_h2_set_plan(const H2Args& a, int bn, int workers, int xcd_groups);
 int split_f16(bool f64, const void* X, int64_t ldx, int64_t rows, int64_t cols,
               int64_t rows_pad, int64_t cols_pad, float scale, bool transpose, _Float16* hi,
               _Float16* lo, int64_t ldo, hipStream_t s);

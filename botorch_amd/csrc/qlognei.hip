@@ -434,6 +434,62 @@ size_t qlognei_ws_bytes(const bta_gp_model* m, const bta_qlognei_state* st, int6
 }
 
 // ---------------------------------------------------------------------------
+// Measured tuning of the step's split-FP16 products: every plan candidate of
+// the forward product (tile width, worker count, XCD groups) is timed inside
+// the whole step with HIP events, the fastest kept, then the same for the
+// adjoint product (which decides whether the adjoint and the MC stage share a
+// launch: they do for 64-column plans).  The winners replace the cost model's
+// plans for these problem shapes.
+// ---------------------------------------------------------------------------
+int qlognei_autotune(const bta_gp_model* m, const bta_qlognei_state* st, const double* X,
+                     int64_t b, double* value, double* grad, int* status, void* ws,
+                     size_t ws_bytes, int* chosen, hipStream_t s) {
+  StepArgs A;
+  int rc = qlognei_step_args(m, st, X, b, value, grad, status, ws, ws_bytes, A);
+  if (rc) return rc;
+  for (int i = 0; i < 6; ++i) chosen[i] = 0;
+  if (!A.split) return BTA_OK;  // the FP64 and FP32 products keep the cost model's plans
+  hipEvent_t e0, e1;
+  BTA_HIP(hipEventCreate(&e0));
+  BTA_HIP(hipEventCreate(&e1));
+  auto time_step = [&](float* ms) -> int {
+    int r = qlognei_value_grad(m, st, X, b, value, grad, status, ws, ws_bytes, s);  // warm
+    if (r) return r;
+    if (hipEventRecord(e0, s) != hipSuccess) return BTA_ERR_HIP;
+    for (int k = 0; k < 3 && !r; ++k)
+      r = qlognei_value_grad(m, st, X, b, value, grad, status, ws, ws_bytes, s);
+    if (r) return r;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+      return BTA_ERR_HIP;
+    return BTA_OK;
+  };
+  const H2Args* prob[2] = {&A.fwd, &A.adj};
+  for (int p = 0; p < (A.grad ? 2 : 1) && !rc; ++p) {
+    int cand[3 * 32];
+    const int nc = gemm_h2_candidates(*prob[p], cand, 32);
+    float best = 1e30f;
+    int bi = -1;
+    for (int c = 0; c < nc && !rc; ++c) {
+      gemm_h2_set_plan(*prob[p], cand[3 * c], cand[3 * c + 1], cand[3 * c + 2]);
+      float ms = 0.0f;
+      rc = time_step(&ms);
+      if (!rc && ms < best) {
+        best = ms;
+        bi = c;
+      }
+    }
+    if (!rc && bi >= 0) {
+      gemm_h2_set_plan(*prob[p], cand[3 * bi], cand[3 * bi + 1], cand[3 * bi + 2]);
+      for (int i = 0; i < 3; ++i) chosen[3 * p + i] = cand[3 * bi + i];
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // Allocation-free variant of the step: its workspace is `scratch`, a static
 // device array (one copy on every GPU the library runs on, held until the
 // process ends).  Every kernel of the step writes the parts of its workspace
@@ -508,6 +564,18 @@ extern "C" int bta_qlognei_value_grad_noalloc(const bta_gp_model* m,
                                               int32_t* status, bta_stream_t stream) {
   return bta::qlognei_value_grad_noalloc(m, st, X, b, value, grad, status,
                                          (hipStream_t)stream);
+}
+
+extern "C" int bta_qlognei_autotune(const bta_gp_model* m, const bta_qlognei_state* st,
+                                    const double* X, int64_t b, double* value, double* grad,
+                                    int32_t* status, void* ws, size_t ws_bytes, int32_t* chosen,
+                                    bta_stream_t stream) {
+  if (!chosen) {
+    bta::set_error("qlognei_autotune: chosen (6 host ints) is NULL");
+    return BTA_ERR_INVALID;
+  }
+  return bta::qlognei_autotune(m, st, X, b, value, grad, status, ws, ws_bytes, chosen,
+                               (hipStream_t)stream);
 }
 
 extern "C" int bta_qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st,

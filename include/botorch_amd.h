@@ -296,6 +296,18 @@ int bta_qlognei_prepare(const bta_gp_model* m, const bta_qlognei_state* st,
 size_t bta_qlognei_workspace_size(const bta_gp_model* m, const bta_qlognei_state* st,
                                   int64_t b);
 
+/* Measured tuner (no counterpart in the reference: the project's own autotuner).
+ * For BTA_PREC_MIXED, times every plan candidate (tile width, worker count, XCD
+ * groups) of the step's forward and then adjoint split-FP16 product inside
+ * whole value(+grad) steps on these arguments (HIP events), keeps the fastest for
+ * these problem shapes, and writes them to chosen (host, 6 ints: bn, workers,
+ * groups of the forward and of the adjoint product; zeros for other precisions,
+ * whose products keep the cost model's plans).  The outputs hold a valid step
+ * afterwards.  SYNCHRONISES; not for graph capture. */
+int bta_qlognei_autotune(const bta_gp_model* m, const bta_qlognei_state* st, const double* X,
+                         int64_t b, double* value, double* grad, int32_t* status, void* ws,
+                         size_t ws_bytes, int32_t* chosen, bta_stream_t stream);
+
 /* value[b] = qLogNEI(X[b]) for X: b x q x d (unscaled); grad (b x q x d, may be
  * NULL for value only) = d value[b] / d X[b].  status[b]: BTA_STATUS_* bits. */
 int bta_qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st,

@@ -235,3 +235,22 @@ def test_mc_lds_limit_same_for_value_only_and_value_grad(S, fits):
         for need_grad in (False, True):
             with pytest.raises(B.UnsupportedError):
                 acq._launch(Xd, need_grad)
+
+
+
+def test_autotune_keeps_results_and_picks_a_plan():
+    """bta_qlognei_autotune times the split GEMMs' plan candidates inside whole
+    steps and keeps the fastest; every plan computes each output element in the
+    same order, so the step's results are bit-identical before and after."""
+    _say("autotune n=2048 d=12 q=4 S=512 b=512 mixed")
+    acq, _, _ = _acq_pair(2048, 12, 4, 512, "mixed", seed=0)
+    X = torch.rand(512, 4, 12, generator=torch.Generator().manual_seed(4),
+                   dtype=torch.float64).to(DEV)
+    v0, g0, s0 = acq._launch(X, True)
+    chosen = acq.autotune(X)
+    print(chosen, flush=True)
+    v1, g1, s1 = acq._launch(X, True)
+    torch.cuda.synchronize()
+    assert torch.equal(v0, v1) and torch.equal(g0, g1) and torch.equal(s0, s1)
+    for p in ("forward", "adjoint"):
+        assert chosen[p]["bn"] in (64, 128) and chosen[p]["workers"] >= 1
