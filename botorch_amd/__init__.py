@@ -8,7 +8,8 @@ include/botorch_amd.h).
 from .acquisition import (TAU_MAX, TAU_RELU, BatchedQLogNEI, prune_inferior_points,
                           qLogNoisyExpectedImprovement)
 from .exceptions import (BotorchError, BotorchWarning, InputDataError, NanError, NotPSDError,
-                         NumericalWarning, OptimizationGradientError, UnsupportedError)
+                         NumericalWarning, OptimizationGradientError, OptimizationWarning,
+                         UnsupportedError)
 from .gp import (ConstantMean, GaussianLikelihood, GPyTorchPosterior, GreaterThan,
                  LogNormalPrior, MaternKernel, Positive, RBFKernel, ScaleKernel, SingleTaskGP,
                  Standardize, psd_safe_cholesky)

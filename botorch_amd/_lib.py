@@ -148,6 +148,12 @@ SIGNATURES = {
                                              _i64, _f64, _f64, _p, _p]),
     "bta_log_areas_backward": (ctypes.c_int, [_i32, _p, _p, _i64, _i64, _i32, _i32, _p, _p,
                                               _i64, _i64, _f64, _f64, _p, _p]),
+    "bta_lbfgsb_state_size": (_sz, [_i32, _i32]),
+    "bta_lbfgsb_init": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _p, _p, _p, _f64, _f64, _i32,
+                                       _i32, _i32, _p, _p, _p]),
+    "bta_lbfgsb_step": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _p, _p, _p, _p, _f64, _p, _p,
+                                       _p, _p]),
+    "bta_lbfgsb_results": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _p, _p, _p]),
     "bta_mll_workspace_size": (_sz, [_i64, _i32]),
     "bta_mll_value_grad": (ctypes.c_int, [_i32, _p, _p, _i64, _i32, ctypes.POINTER(_f64), _f64,
                                           _f64, _f64, _i32, ctypes.POINTER(_f64), _p, _sz, _p]),

@@ -34,5 +34,9 @@ class BotorchWarning(Warning):
     """Base botorch_amd warning (exceptions/warnings.py:12)."""
 
 
+class OptimizationWarning(BotorchWarning):
+    """Optimisation-related warnings (exceptions/warnings.py:38)."""
+
+
 class NumericalWarning(RuntimeWarning):
     """Jitter added to a factorisation (linear_operator NumericalWarning)."""

@@ -404,6 +404,46 @@ int bta_qlognei_value_grad_noalloc(const bta_gp_model* m, const bta_qlognei_stat
                                    int32_t* status, bta_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Lock-step batched L-BFGS-B on the device.  Replaces the per-restart SciPy  */
+/* setulb loop of optim/batched_lbfgs_b.py:440-575 (fmin_l_bfgs_b_batched,    */
+/* driven from generation/gen.py:236-249): b independent problems of n        */
+/* variables, m correction pairs, L-BFGS-B 3.0 steps and the reference's      */
+/* stopping rules (factr, pgtol, maxls, maxiter; maxfun as the library's     */
+/* host driver).  All arrays are device memory, row-major b x n.            */
+/* ------------------------------------------------------------------------ */
+
+/* Bytes of one problem's state (multiply by b); 0 when n, m do not fit the
+ * kernel's LDS budget (64 KiB per problem: n <= 224 at m = 10, 471 at m = 3). */
+size_t bta_lbfgsb_state_size(int32_t n, int32_t m);
+
+/* Starts b problems at x0 (projected onto the bounds): nbd[i] = 0 unbounded,
+ * 1 lower only, 2 both, 3 upper only (SciPy's convention; lower/upper values
+ * are ignored where absent).  Writes the first points to evaluate into x_eval
+ * and zeroes *ndone (may be NULL). */
+int bta_lbfgsb_init(void* state, int64_t b, int32_t n, int32_t m, const double* x0,
+                    const double* lower, const double* upper, const int32_t* nbd, double factr,
+                    double pgtol, int32_t maxls, int32_t maxiter, int32_t maxfun,
+                    double* x_eval, int32_t* ndone, bta_stream_t stream);
+
+/* One round: f (b) and g (b x n) are the objective and gradient at x_eval,
+ * both scaled by fscale (-1 to minimise -acqf); step_status (b, may be NULL)
+ * is OR-ed into each problem's record.  Advances every running problem to its
+ * next evaluation or its end and writes the next points into x_eval; ended
+ * problems keep their final x there and add 1 to *ndone once.  A NaN in a
+ * gradient ends that problem (task 6). */
+int bta_lbfgsb_step(void* state, int64_t b, int32_t n, int32_t m, const double* lower,
+                    const double* upper, const int32_t* nbd, const double* f, const double* g,
+                    double fscale, const int32_t* step_status, double* x_eval, int32_t* ndone,
+                    bta_stream_t stream);
+
+/* x (b x n), f (b, in the minimised scale) and info (b x 6: iterations,
+ * evaluations, warnflag 0 converged / 1 maxiter or maxfun / 2 other, task
+ * 1 pgtol / 2 factr / 3 abnormal line search / 4 maxiter / 5 maxfun /
+ * 6 NaN gradient, OR of the step statuses, pair updates). */
+int bta_lbfgsb_results(const void* state, int64_t b, int32_t n, int32_t m, double* x, double* f,
+                       int32_t* info, bta_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Exact marginal log likelihood value + adjoint.  Replaces gpytorch          */
 /* ExactMarginalLogLikelihood forward + torch autograd backward as driven by */
 /* optim/closures/model_closures.py:125-145 and fit.py:76-289.               */

@@ -63,3 +63,61 @@ def test_constraint_violating_hyperparameter_raises():
         lik.noise = 1e-6
     lik.noise = 1e-4  # the bound itself is allowed
     assert abs(float(lik.noise.detach()) - 1e-4) < 1e-12
+
+
+class _WrongGradAcqf:
+    """Host stand-in for an acquisition function whose gradient has the wrong
+    sign: every line search fails, so each restart ends abnormally and
+    gen_candidates_scipy raises an OptimizationWarning."""
+
+    def __init__(self, sign=-1.0):
+        self.X_baseline = torch.zeros(1, 2, dtype=torch.float64)
+        self.sign = sign
+        self.calls = 0
+
+    def __call__(self, X):
+        return -((X - 0.3) ** 2).sum(dim=(-1, -2))
+
+    def value_and_grad(self, X):
+        self.calls += 1
+        return self(X), self.sign * (-2.0) * (X - 0.3)
+
+
+def test_optimize_acqf_retries_on_optimization_warning():
+    """optim/optimize.py:476-526: an OptimizationWarning from the restarts
+    draws new initial conditions and retries once; a second failure warns
+    again; retry_on_optimization_warning=False keeps the first result."""
+    import warnings
+
+    from botorch_amd.optim import optimize_acqf
+    bounds = torch.tensor([[0.0, 0.0], [1.0, 1.0]], dtype=torch.float64)
+    acq = _WrongGradAcqf()
+    with warnings.catch_warnings(record=True) as ws:
+        warnings.simplefilter("always")
+        optimize_acqf(acq, bounds, q=1, num_restarts=3, raw_samples=16)
+    msgs = [str(w.message) for w in ws if issubclass(w.category, RuntimeWarning)]
+    assert any("Trying again with a new set of initial conditions" in m for m in msgs)
+    assert any("Optimization failed on the second try" in m for m in msgs)
+    first = acq.calls
+    acq2 = _WrongGradAcqf()
+    with warnings.catch_warnings(record=True) as ws:
+        warnings.simplefilter("always")
+        optimize_acqf(acq2, bounds, q=1, num_restarts=3, raw_samples=16,
+                      retry_on_optimization_warning=False)
+    assert not [w for w in ws if issubclass(w.category, RuntimeWarning)]
+    assert acq2.calls * 2 == first
+    # provided initial conditions covering num_restarts: no new draw, one warning
+    acq3 = _WrongGradAcqf()
+    ics = torch.rand(3, 1, 2, dtype=torch.float64)
+    with warnings.catch_warnings(record=True) as ws:
+        warnings.simplefilter("always")
+        optimize_acqf(acq3, bounds, q=1, num_restarts=3, batch_initial_conditions=ics)
+    msgs = [str(w.message) for w in ws if issubclass(w.category, RuntimeWarning)]
+    assert len(msgs) == 1 and "will not be retried" in msgs[0]
+    # a correct gradient converges without warnings or retries
+    ok = _WrongGradAcqf(sign=1.0)
+    with warnings.catch_warnings(record=True) as ws:
+        warnings.simplefilter("always")
+        cand, val = optimize_acqf(ok, bounds, q=1, num_restarts=3, raw_samples=16)
+    assert not [w for w in ws if issubclass(w.category, RuntimeWarning)]
+    assert torch.allclose(cand, torch.full((1, 2), 0.3, dtype=torch.float64), atol=1e-5)
