@@ -61,6 +61,69 @@ struct WaveLanes {
     return __popcll(b & ((1ull << threadIdx.x) - 1ull));
   }
   __device__ void sync() const { __syncthreads(); }
+  __device__ void mark(int) const {}
+  // v of lane j (j uniform)
+  __device__ double bcast(double v, int j) const {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+  }
+  // upper Cholesky row by row: row j's diagonal from the column above it, then
+  // the rest of row j across lanes (serial_chol_upper's operations and order)
+  __device__ int chol_upper(double* A, int ld, int k) const {
+    const int l0 = lane();
+    for (int j = 0; j < k; ++j) {
+      double s = 0.0;
+      for (int q = 0; q < j; ++q) s += A[q * ld + j] * A[q * ld + j];
+      s = A[j * ld + j] - s;
+      __syncthreads();
+      if (!(s > 0.0)) return j + 1;
+      const double ujj = sqrt(s);
+      if (l0 == 0) A[j * ld + j] = ujj;
+      for (int l = j + 1 + l0; l < k; l += W) {
+        double t = A[j * ld + l];
+        for (int i = 0; i < j; ++i) t -= A[i * ld + j] * A[i * ld + l];
+        A[j * ld + l] = t / ujj;
+      }
+      __syncthreads();
+    }
+    return 0;
+  }
+  // R^T y = b: lane i holds b_i; step j finishes y_j and removes it from the
+  // later entries (serial_solve_rt's operations and order)
+  __device__ int solve_rt(const double* R, int ld, int k, double* b) const {
+    if (k > W) return lbfgsb::serial_solve_rt(R, ld, k, b);
+    for (int j = 0; j < k; ++j)
+      if (R[j * ld + j] == 0.0) return j + 1;
+    const int l0 = lane();
+    double bi = l0 < k ? b[l0] : 0.0;
+    for (int j = 0; j < k; ++j) {
+      const double yj = bcast(bi, j) / R[j * ld + j];
+      if (l0 == j) bi = yj;
+      else if (l0 > j && l0 < k) bi -= R[j * ld + l0] * yj;
+    }
+    __syncthreads();
+    if (l0 < k) b[l0] = bi;
+    __syncthreads();
+    return 0;
+  }
+  // R y = b from the last column (serial_solve_r's operations and order)
+  __device__ int solve_r(const double* R, int ld, int k, double* b) const {
+    if (k > W) return lbfgsb::serial_solve_r(R, ld, k, b);
+    for (int j = 0; j < k; ++j)
+      if (R[j * ld + j] == 0.0) return j + 1;
+    const int l0 = lane();
+    double bi = l0 < k ? b[l0] : 0.0;
+    for (int j = k - 1; j >= 0; --j) {
+      const double yj = bcast(bi, j) / R[j * ld + j];
+      if (l0 == j) bi = yj;
+      else if (l0 < j) bi -= R[l0 * ld + j] * yj;
+    }
+    __syncthreads();
+    if (l0 < k) b[l0] = bi;
+    __syncthreads();
+    return 0;
+  }
 };
 
 struct LbArgs {

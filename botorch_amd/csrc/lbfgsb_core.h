@@ -72,6 +72,14 @@ enum Task : int32_t {
 
 enum LsTask : int32_t { LS_START = 0, LS_FG = 1, LS_CONV = 2, LS_WARN = 3, LS_ERROR = 4 };
 
+// phases a lanes context may time through mark() (both library contexts ignore
+// the marks): the generalized Cauchy point, the free/active split, the
+// subspace factorisation, the reduced gradient and subspace step, the line
+// search and the pair update
+enum Phase : int32_t {
+  PH_OTHER = 0, PH_CAUCHY, PH_FREEV, PH_FORMK, PH_SUBSM, PH_LNSRCH, PH_UPDATE, PH_COUNT
+};
+
 // the scalar state of one problem (first bytes of its slab)
 struct Scal {
   double f, fold, theta, stp, stpmx, gd, gdold, dtd, dnorm, sbgnrm;
@@ -91,7 +99,7 @@ struct Scal {
 struct Layout {
   int n, m;
   int o_x, o_g, o_t, o_r, o_d, o_z, o_ws, o_wy, o_sy, o_ss, o_wt, o_wn1, o_iw, o_free, slab;
-  int o_wn, o_p, o_c, o_v, o_wbp, o_wv, o_rd, o_xp, o_bt, o_bi, o_ef, total;
+  int o_wn, o_p, o_c, o_v, o_wbp, o_wv, o_rd, o_xp, o_bt, o_bi, o_lists, total;
   __host__ __device__ Layout(int n_, int m_) : n(n_), m(m_) {
     int o = (int)((sizeof(Scal) + 7) / 8);
     const int ni = (n + 1) / 2;  // n int32 in doubles
@@ -120,7 +128,7 @@ struct Layout {
     o_xp = o; o += n;
     o_bt = o; o += n;
     o_bi = o; o += ni;
-    o_ef = o; o += ni;
+    o_lists = o; o += 4 * ni;
     total = o;
   }
 };
@@ -131,7 +139,10 @@ struct View {
   double *x, *g, *t, *r, *d, *z, *ws, *wy, *sy, *ss, *wt, *wn1;
   int32_t *iwhere, *freeprev;
   double *wn, *p, *c, *v, *wbp, *wv, *rd, *xp, *bt;
-  int32_t *bi, *ef;
+  int32_t *bi;
+  // variables at the Cauchy point, increasing: free, active, entered and left
+  // the free set since the previous split (counts nfree, n - nfree, nenter, nleave)
+  int32_t *lfree, *lact, *lent, *llv;
   const double* lo;
   const double* up;
   const int32_t* nbd;
@@ -148,19 +159,24 @@ __host__ __device__ inline View make_view(double* base, const Layout& L, const d
   w.iwhere = (int32_t*)(base + L.o_iw); w.freeprev = (int32_t*)(base + L.o_free);
   w.wn = base + L.o_wn; w.p = base + L.o_p; w.c = base + L.o_c; w.v = base + L.o_v;
   w.wbp = base + L.o_wbp; w.wv = base + L.o_wv; w.rd = base + L.o_rd; w.xp = base + L.o_xp;
-  w.bt = base + L.o_bt; w.bi = (int32_t*)(base + L.o_bi); w.ef = (int32_t*)(base + L.o_ef);
+  w.bt = base + L.o_bt; w.bi = (int32_t*)(base + L.o_bi);
+  w.lfree = (int32_t*)(base + L.o_lists);
+  w.lact = w.lfree + L.n;
+  w.lent = w.lact + L.n;
+  w.llv = w.lent + L.n;
   w.lo = lo; w.up = up; w.nbd = nbd;
   w.n = L.n; w.m = L.m;
   return w;
 }
 
 // ---------------------------------------------------------------------------
-// small dense helpers (uniform: every lane runs them on the same data)
+// small dense helpers, serial forms (the host context runs these; the device
+// context has lane-parallel forms with the same operation order)
 
 // in-place upper Cholesky A = R^T R of the leading k x k block (row-major,
 // leading dimension ld), column by column; returns the 1-based index of the
 // first non-positive pivot, 0 on success
-__host__ __device__ inline int chol_upper(double* A, int ld, int k) {
+__host__ __device__ inline int serial_chol_upper(double* A, int ld, int k) {
   for (int j = 0; j < k; ++j) {
     double s = 0.0;
     for (int q = 0; q < j; ++q) {
@@ -168,8 +184,8 @@ __host__ __device__ inline int chol_upper(double* A, int ld, int k) {
       for (int i = 0; i < q; ++i) t -= A[i * ld + q] * A[i * ld + j];
       t /= A[q * ld + q];
       A[q * ld + j] = t;
-      s += t * t;
     }
+    for (int q = 0; q < j; ++q) s += A[q * ld + j] * A[q * ld + j];
     s = A[j * ld + j] - s;
     if (!(s > 0.0)) return j + 1;
     A[j * ld + j] = sqrt(s);
@@ -177,8 +193,9 @@ __host__ __device__ inline int chol_upper(double* A, int ld, int k) {
   return 0;
 }
 
-// solve R^T y = b in place (R upper k x k, forward substitution)
-__host__ __device__ inline int solve_rt(const double* R, int ld, int k, double* b) {
+// solve R^T y = b in place (R upper k x k): y_j = (b_j - sum_{i<j} R_ij y_i) / R_jj,
+// the terms subtracted in increasing i
+__host__ __device__ inline int serial_solve_rt(const double* R, int ld, int k, double* b) {
   for (int j = 0; j < k; ++j)
     if (R[j * ld + j] == 0.0) return j + 1;
   for (int j = 0; j < k; ++j) {
@@ -189,45 +206,76 @@ __host__ __device__ inline int solve_rt(const double* R, int ld, int k, double* 
   return 0;
 }
 
-// solve R y = b in place (R upper k x k, back substitution)
-__host__ __device__ inline int solve_r(const double* R, int ld, int k, double* b) {
+// solve R y = b in place (R upper k x k) column by column from the last: y_j =
+// b_j / R_jj, then b_i -= R_ij y_j for i < j (LINPACK dtrsl / LAPACK dtrsm order)
+__host__ __device__ inline int serial_solve_r(const double* R, int ld, int k, double* b) {
   for (int j = 0; j < k; ++j)
     if (R[j * ld + j] == 0.0) return j + 1;
   for (int j = k - 1; j >= 0; --j) {
-    double t = b[j];
-    for (int i = j + 1; i < k; ++i) t -= R[j * ld + i] * b[i];
-    b[j] = t / R[j * ld + j];
+    b[j] = b[j] / R[j * ld + j];
+    const double yj = b[j];
+    for (int i = 0; i < j; ++i) b[i] -= R[i * ld + j] * yj;
   }
   return 0;
 }
 
+// the one-lane context: every loop serial, the small dense operations in their
+// serial forms (the host build of the machine)
+struct SerialLanes {
+  static constexpr int W = 1;
+  __host__ __device__ int lane() const { return 0; }
+  __host__ __device__ double sum(double v) const { return v; }
+  __host__ __device__ double max(double v) const { return v; }
+  __host__ __device__ double min(double v) const { return v; }
+  __host__ __device__ int sum_int(int v) const { return v; }
+  __host__ __device__ int min_int(int v) const { return v; }
+  __host__ __device__ int any(int v) const { return v != 0; }
+  __host__ __device__ int ballot_rank(int flag, int& total) const {
+    total = flag ? 1 : 0;
+    return 0;
+  }
+  __host__ __device__ void sync() const {}
+  __host__ __device__ void mark(int) const {}
+  __host__ __device__ int chol_upper(double* A, int ld, int k) const {
+    return serial_chol_upper(A, ld, k);
+  }
+  __host__ __device__ int solve_rt(const double* R, int ld, int k, double* b) const {
+    return serial_solve_rt(R, ld, k, b);
+  }
+  __host__ __device__ int solve_r(const double* R, int ld, int k, double* b) const {
+    return serial_solve_r(R, ld, k, b);
+  }
+};
+
 // p = M v for the 2col x 2col middle matrix M of B = theta I - W M W^T,
 // W = [Y, theta S]: M^{-1} = [[-D, L^T], [L, theta S^T S]] with D = diag(S^T Y)
 // and L its strictly lower part.  wt holds the upper factor J^T of
-// T = theta S^T S + L D^{-1} L^T = J J^T.  Nonzero return: singular factor.
-__host__ __device__ inline int bmv(const View& w, int col, const double* v, double* p) {
+// T = theta S^T S + L D^{-1} L^T = J J^T.  v and p are shared arrays.
+// Nonzero return: singular factor.
+template <class C>
+__host__ __device__ inline int bmv(C& c, const View& w, int col, const double* v, double* p) {
   if (col == 0) return 0;
   const int m = w.m;
   const double* sy = w.sy;
-  // p2 = J^{-1} (v2 + L D^{-1} v1)
-  p[col] = v[col];
-  for (int i = 1; i < col; ++i) {
+  // p2 = v2 + L D^{-1} v1 and p1 = D^{-1/2} v1
+  for (int i = c.lane(); i < col; i += C::W) {
     double sum = 0.0;
     for (int k = 0; k < i; ++k) sum += sy[i * m + k] * v[k] / sy[k * m + k];
-    p[col + i] = v[col + i] + sum;
+    p[col + i] = i == 0 ? v[col] : v[col + i] + sum;
+    p[i] = v[i] / sqrt(sy[i * m + i]);
   }
-  if (solve_rt(w.wt, m, col, p + col)) return 1;
-  // p1 = D^{-1/2} v1
-  for (int i = 0; i < col; ++i) p[i] = v[i] / sqrt(sy[i * m + i]);
-  // p2 = J^{-T} p2
-  if (solve_r(w.wt, m, col, p + col)) return 1;
+  c.sync();
+  // p2 = J^{-1} p2, then J^{-T} p2
+  if (c.solve_rt(w.wt, m, col, p + col)) return 1;
+  if (c.solve_r(w.wt, m, col, p + col)) return 1;
   // p1 = -D^{-1/2} p1 + D^{-1} L^T p2
-  for (int i = 0; i < col; ++i) p[i] = -p[i] / sqrt(sy[i * m + i]);
-  for (int i = 0; i < col; ++i) {
+  for (int i = c.lane(); i < col; i += C::W) {
+    const double pi = -p[i] / sqrt(sy[i * m + i]);
     double sum = 0.0;
     for (int k = i + 1; k < col; ++k) sum += sy[k * m + i] * p[col + k] / sy[i * m + i];
-    p[i] += sum;
+    p[i] = pi + sum;
   }
+  c.sync();
   return 0;
 }
 
@@ -529,7 +577,7 @@ __host__ __device__ inline int cauchy(C& c, View& w) {
   Scal& s = *w.s;
   const int n = w.n, m = w.m, col = s.col, col2 = 2 * col;
   const double theta = s.theta;
-  for (int k = 0; k < col2; ++k) w.c[k] = 0.0;
+  for (int k = c.lane(); k < col2; k += C::W) w.c[k] = 0.0;
   if (s.sbgnrm <= 0.0) {
     for (int i = c.lane(); i < n; i += C::W) w.z[i] = w.x[i];
     c.sync();
@@ -606,10 +654,10 @@ __host__ __device__ inline int cauchy(C& c, View& w) {
   double f2 = -theta * f1;
   const double f2_org = f2;
   if (col > 0) {
-    if (bmv(w, col, w.p, w.v)) return -1;
+    if (bmv(c, w, col, w.p, w.v)) return -1;
     double vp = 0.0;
-    for (int k = 0; k < col2; ++k) vp += w.v[k] * w.p[k];
-    f2 -= vp;
+    for (int k = c.lane(); k < col2; k += C::W) vp += w.v[k] * w.p[k];
+    f2 -= c.sum(vp);
   }
   double dtm = -f1 / f2, tsum = 0.0;
   bool all_fixed = false;
@@ -664,20 +712,26 @@ __host__ __device__ inline int cauchy(C& c, View& w) {
       f1 = f1 + dt * f2 + dibp2 - theta * dibp * zibp;
       f2 = f2 - theta * dibp2;
       if (col > 0) {
-        for (int k = 0; k < col2; ++k) w.c[k] += dt * w.p[k];
-        for (int j = 0; j < col; ++j) {
-          const int slot = (s.head + j) % m;
-          w.wbp[j] = w.wy[(size_t)slot * n + ibp];
-          w.wbp[col + j] = theta * w.ws[(size_t)slot * n + ibp];
+        // c += dt p; wbp = the row of W = [Y, theta S] of the breakpoint's variable
+        for (int k = c.lane(); k < col2; k += C::W) {
+          w.c[k] += dt * w.p[k];
+          const int slot = (s.head + (k < col ? k : k - col)) % m;
+          w.wbp[k] = k < col ? w.wy[(size_t)slot * n + ibp]
+                             : theta * w.ws[(size_t)slot * n + ibp];
         }
-        if (bmv(w, col, w.wbp, w.v)) return -1;
+        c.sync();
+        if (bmv(c, w, col, w.wbp, w.v)) return -1;
         double wmc = 0.0, wmp = 0.0, wmw = 0.0;
-        for (int k = 0; k < col2; ++k) {
+        for (int k = c.lane(); k < col2; k += C::W) {
           wmc += w.v[k] * w.c[k];
           wmp += w.v[k] * w.p[k];
           wmw += w.v[k] * w.wbp[k];
+          w.p[k] -= dibp * w.wbp[k];
         }
-        for (int k = 0; k < col2; ++k) w.p[k] -= dibp * w.wbp[k];
+        wmc = c.sum(wmc);
+        wmp = c.sum(wmp);
+        wmw = c.sum(wmw);
+        c.sync();
         f1 += dibp * wmc;
         f2 += 2.0 * dibp * wmp - dibp2 * wmw;
       }
@@ -702,36 +756,46 @@ __host__ __device__ inline int cauchy(C& c, View& w) {
     tsum += dtm;
     for (int i = c.lane(); i < n; i += C::W) w.z[i] += tsum * w.d[i];
   }
-  if (col > 0)
-    for (int k = 0; k < col2; ++k) w.c[k] += dtm * w.p[k];
+  for (int k = c.lane(); k < col2; k += C::W) w.c[k] += dtm * w.p[k];
   c.sync();
   return 0;
 }
 
-// free set at the Cauchy point (iwhere <= 0): its size, and the variables that
-// entered (ef = 1) or left (ef = 2) it since the previous split (counted from
-// the second iteration on, constrained problems only)
+// free set at the Cauchy point (iwhere <= 0) and its complement as index
+// lists, and the variables that entered or left the free set since the
+// previous split (counted from the second iteration on, constrained problems
+// only); all lists in increasing index
 template <class C>
 __host__ __device__ inline void freev(C& c, View& w) {
   Scal& s = *w.s;
   const bool count = s.iter > 0 && s.cnstnd;
-  int nf = 0, ne = 0, nl = 0;
-  for (int i = c.lane(); i < w.n; i += C::W) {
-    const int fr = w.iwhere[i] <= 0;
-    int e = 0;
-    if (count) {
-      if (w.freeprev[i] && !fr) e = 2;
-      else if (!w.freeprev[i] && fr) e = 1;
+  int nf = 0, na = 0, ne = 0, nl = 0;
+  for (int base = 0; base < w.n; base += C::W) {
+    const int i = base + c.lane();
+    const bool in = i < w.n;
+    const int fr = in && w.iwhere[i] <= 0;
+    int ent = 0, lv = 0;
+    if (in) {
+      if (count) {
+        lv = w.freeprev[i] && !fr;
+        ent = !w.freeprev[i] && fr;
+      }
+      w.freeprev[i] = fr;
     }
-    w.ef[i] = e;
-    w.freeprev[i] = fr;
-    nf += fr;
-    ne += e == 1;
-    nl += e == 2;
+    int t;
+    int pos = c.ballot_rank(fr, t);
+    if (fr) w.lfree[nf + pos] = i;
+    nf += t;
+    pos = c.ballot_rank(in && !fr, t);
+    if (in && !fr) w.lact[na + pos] = i;
+    na += t;
+    pos = c.ballot_rank(ent, t);
+    if (ent) w.lent[ne + pos] = i;
+    ne += t;
+    pos = c.ballot_rank(lv, t);
+    if (lv) w.llv[nl + pos] = i;
+    nl += t;
   }
-  nf = c.sum_int(nf);
-  ne = c.sum_int(ne);
-  nl = c.sum_int(nl);
   c.sync();
   s.nfree = nf;
   s.nenter = ne;
@@ -739,19 +803,16 @@ __host__ __device__ inline void freev(C& c, View& w) {
   c.sync();
 }
 
-// sum over the variables k with sel(k) of a[k] b[k], in increasing k (up) or
-// decreasing k (the order in which the published code lists active, entering
-// and leaving variables)
-template <class Sel>
-__host__ __device__ inline double masked_dot(const double* a, const double* b, int n, bool up,
-                                             Sel sel) {
+// sum over the listed variables of a[k] b[k], in increasing index (up) or
+// decreasing index (the order in which the published code walks its lists of
+// active, entering and leaving variables)
+__host__ __device__ inline double list_dot(const double* a, const double* b,
+                                           const int32_t* list, int cnt, bool up) {
   double acc = 0.0;
   if (up) {
-    for (int k = 0; k < n; ++k)
-      if (sel(k)) acc += a[k] * b[k];
+    for (int t = 0; t < cnt; ++t) acc += a[list[t]] * b[list[t]];
   } else {
-    for (int k = n - 1; k >= 0; --k)
-      if (sel(k)) acc += a[k] * b[k];
+    for (int t = cnt - 1; t >= 0; --t) acc += a[list[t]] * b[list[t]];
   }
   return acc;
 }
@@ -771,21 +832,27 @@ __host__ __device__ inline int formk(C& c, View& w) {
   const int n = w.n, m = w.m, col = s.col, ld = 2 * m;
   const double theta = s.theta;
   double* wn1 = w.wn1;
-  const int32_t* iw = w.iwhere;
-  const int32_t* ef = w.ef;
-  auto is_free = [iw](int k) { return iw[k] <= 0; };
-  auto is_act = [iw](int k) { return iw[k] > 0; };
-  auto entered = [ef](int k) { return ef[k] == 1; };
-  auto left = [ef](int k) { return ef[k] == 2; };
+  const int nf = s.nfree, na = n - s.nfree, ne = s.nenter, nl = s.nleave;
   int upcl = col;
   if (s.updatd) {
     if (s.iupdat > m) {
-      // the oldest pair was dropped: shift the old part of WN1
-      for (int jy = 0; jy < m - 1; ++jy) {
-        const int js = m + jy;
-        for (int i = 0; i < m - 1 - jy; ++i) wn1[(jy + i) * ld + jy] = wn1[(jy + 1 + i) * ld + jy + 1];
-        for (int i = 0; i < m - 1 - jy; ++i) wn1[(js + i) * ld + js] = wn1[(js + 1 + i) * ld + js + 1];
-        for (int i = 0; i < m - 1; ++i) wn1[(m + i) * ld + jy] = wn1[(m + 1 + i) * ld + jy + 1];
+      // the oldest pair was dropped: entry (r, q) of the old part of WN1 - the
+      // lower triangles of the diagonal blocks and the (2,1) block, r, q below
+      // the last row / column of their block - takes entry (r + 1, q + 1)
+      // (every source is read before any target is written)
+      // (wn, rewritten below, holds the sources)
+      const int m1 = m - 1;
+      double* tmp = w.wn;
+      for (int e = c.lane(); e < 3 * m1 * m1; e += C::W) {
+        const int blk = e / (m1 * m1), rq = e % (m1 * m1), r = rq / m1, q = rq % m1;
+        const int R = blk == 0 ? r : m + r, Q = blk == 1 ? m + q : q;
+        tmp[e] = (blk == 2 || r >= q) ? wn1[(R + 1) * ld + Q + 1] : 0.0;
+      }
+      c.sync();
+      for (int e = c.lane(); e < 3 * m1 * m1; e += C::W) {
+        const int blk = e / (m1 * m1), rq = e % (m1 * m1), r = rq / m1, q = rq % m1;
+        const int R = blk == 0 ? r : m + r, Q = blk == 1 ? m + q : q;
+        if (blk == 2 || r >= q) wn1[R * ld + Q] = tmp[e];
       }
     }
     c.sync();
@@ -798,13 +865,13 @@ __host__ __device__ inline int formk(C& c, View& w) {
       const double* yj = w.wy + (size_t)pj * n;
       const double* sj = w.ws + (size_t)pj * n;
       if (kind == 0) {
-        wn1[iy * ld + jy] = masked_dot(ynew, yj, n, true, is_free);
+        wn1[iy * ld + jy] = list_dot(ynew, yj, w.lfree, nf, true);
       } else if (kind == 1) {
-        wn1[(m + iy) * ld + m + jy] = masked_dot(snew, sj, n, false, is_act);
+        wn1[(m + iy) * ld + m + jy] = list_dot(snew, sj, w.lact, na, false);
       } else if (kind == 2) {
-        if (jy < col - 1) wn1[(m + iy) * ld + jy] = masked_dot(snew, yj, n, false, is_act);
+        if (jy < col - 1) wn1[(m + iy) * ld + jy] = list_dot(snew, yj, w.lact, na, false);
       } else {
-        wn1[(m + jy) * ld + iy] = masked_dot(sj, ynew, n, true, is_free);
+        wn1[(m + jy) * ld + iy] = list_dot(sj, ynew, w.lfree, nf, true);
       }
     }
     c.sync();
@@ -820,16 +887,16 @@ __host__ __device__ inline int formk(C& c, View& w) {
       const double* si = w.ws + (size_t)pi * n;
       const double* sj = w.ws + (size_t)pj * n;
       if (jy <= iy) {
-        const double t1 = masked_dot(yi, yj, n, false, entered);
-        const double t2 = masked_dot(si, sj, n, false, entered);
-        const double t3 = masked_dot(yi, yj, n, false, left);
-        const double t4 = masked_dot(si, sj, n, false, left);
+        const double t1 = list_dot(yi, yj, w.lent, ne, false);
+        const double t2 = list_dot(si, sj, w.lent, ne, false);
+        const double t3 = list_dot(yi, yj, w.llv, nl, false);
+        const double t4 = list_dot(si, sj, w.llv, nl, false);
         wn1[iy * ld + jy] = wn1[iy * ld + jy] + t1 - t3;
         wn1[(m + iy) * ld + m + jy] = wn1[(m + iy) * ld + m + jy] - t2 + t4;
       }
       // block (2,1): row iy (S), column jy (Y)
-      const double t1 = masked_dot(si, yj, n, false, entered);
-      const double t3 = masked_dot(si, yj, n, false, left);
+      const double t1 = list_dot(si, yj, w.lent, ne, false);
+      const double t3 = list_dot(si, yj, w.llv, nl, false);
       if (iy <= jy)
         wn1[(m + iy) * ld + jy] = wn1[(m + iy) * ld + jy] + t1 - t3;
       else
@@ -855,7 +922,7 @@ __host__ __device__ inline int formk(C& c, View& w) {
     w.wn[i * ld + j] = val;
   }
   c.sync();
-  const int info1 = chol_upper(w.wn, ld, col);
+  const int info1 = c.chol_upper(w.wn, ld, col);
   c.sync();
   if (info1) return -1;
   // (1,2) block: L^{-1} times each of its columns
@@ -877,7 +944,7 @@ __host__ __device__ inline int formk(C& c, View& w) {
     w.wn[is * ld + js] += acc;
   }
   c.sync();
-  const int info2 = chol_upper(w.wn + col * ld + col, ld, col);
+  const int info2 = c.chol_upper(w.wn + col * ld + col, ld, col);
   c.sync();
   if (info2) return -2;
   return 0;
@@ -894,7 +961,7 @@ __host__ __device__ inline int cmprlb(C& c, View& w) {
     c.sync();
     return 0;
   }
-  const int info = bmv(w, col, w.c, w.v);
+  const int info = bmv(c, w, col, w.c, w.v);
   c.sync();
   if (info) return -8;
   const double theta = s.theta;
@@ -925,17 +992,16 @@ __host__ __device__ inline int subsm(C& c, View& w) {
     const int j = k < col ? k : k - col;
     const int slot = (s.head + j) % m;
     const double* wc = (k < col ? w.wy : w.ws) + (size_t)slot * n;
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i)
-      if (w.iwhere[i] <= 0) acc += wc[i] * w.rd[i];
+    const double acc = list_dot(wc, w.rd, w.lfree, s.nfree, true);
     w.wv[k] = k < col ? acc : theta * acc;
   }
   c.sync();
   // wv = K^{-1} wv
-  int info = solve_rt(w.wn, ld, c2, w.wv);
+  int info = c.solve_rt(w.wn, ld, c2, w.wv);
   if (!info) {
-    for (int k = 0; k < col; ++k) w.wv[k] = -w.wv[k];
-    info = solve_r(w.wn, ld, c2, w.wv);
+    for (int k = c.lane(); k < col; k += C::W) w.wv[k] = -w.wv[k];
+    c.sync();
+    info = c.solve_r(w.wn, ld, c2, w.wv);
   }
   c.sync();
   if (info) return info;
@@ -1143,11 +1209,22 @@ __host__ __device__ inline void matupd(C& c, View& w, double rr, double dr) {
   }
   s.theta = rr / dr;
   if (s.iupdat > m) {
-    // drop the oldest pair: shift the upper triangle of S^T S and the lower
-    // triangle of S^T Y by one
-    for (int j = 0; j < col - 1; ++j) {
-      for (int i = 0; i <= j; ++i) w.ss[i * m + j] = w.ss[(i + 1) * m + (j + 1)];
-      for (int i = j; i < col - 1; ++i) w.sy[i * m + j] = w.sy[(i + 1) * m + (j + 1)];
+    // drop the oldest pair: entry (i, j) of the upper triangle of S^T S and of
+    // the lower triangle of S^T Y takes entry (i + 1, j + 1), for i, j < col - 1
+    // (every source is read before any target is written)
+    // (wn, free until the next subspace step, holds the sources)
+    const int c1 = col - 1;
+    double* tmp = w.wn;
+    for (int e = c.lane(); e < c1 * c1; e += C::W) {
+      const int i = e / c1, j = e % c1;
+      tmp[e] = i <= j ? w.ss[(i + 1) * m + j + 1] : 0.0;
+      tmp[c1 * c1 + e] = i >= j ? w.sy[(i + 1) * m + j + 1] : 0.0;
+    }
+    c.sync();
+    for (int e = c.lane(); e < c1 * c1; e += C::W) {
+      const int i = e / c1, j = e % c1;
+      if (i <= j) w.ss[i * m + j] = tmp[e];
+      if (i >= j) w.sy[i * m + j] = tmp[c1 * c1 + e];
     }
   }
   c.sync();
@@ -1185,7 +1262,7 @@ __host__ __device__ inline int formt(C& c, View& w) {
     w.wt[i * m + j] = ddum + theta * w.ss[i * m + j];
   }
   c.sync();
-  const int info = chol_upper(w.wt, m, col);
+  const int info = c.chol_upper(w.wt, m, col);
   c.sync();
   return info ? -3 : 0;
 }
@@ -1256,13 +1333,15 @@ __host__ __device__ inline int machine(C& c, View& w) {
       if (!s.cnstnd && s.col > 0) {
         for (int i = c.lane(); i < n; i += C::W) {
           w.z[i] = w.x[i];
-          w.ef[i] = 0;
+          w.lfree[i] = i;
         }
         c.sync();
+        s.nfree = n;
         s.nenter = 0;
         s.nleave = 0;
         c.sync();
       } else {
+        c.mark(PH_CAUCHY);
         info = cauchy(c, w);
         c.sync();
         if (info) {
@@ -1270,10 +1349,13 @@ __host__ __device__ inline int machine(C& c, View& w) {
           c.sync();
           continue;
         }
+        c.mark(PH_FREEV);
         freev(c, w);
       }
       if (s.nfree != 0 && s.col != 0) {
+        c.mark(PH_FORMK);
         info = formk(c, w);
+        c.mark(PH_SUBSM);
         if (!info) info = cmprlb(c, w);
         if (!info) info = subsm(c, w);
         c.sync();
@@ -1285,12 +1367,14 @@ __host__ __device__ inline int machine(C& c, View& w) {
       }
       for (int i = c.lane(); i < n; i += C::W) w.d[i] = w.z[i] - w.x[i];
       c.sync();
+      c.mark(PH_LNSRCH);
       lnsrch_start(c, w);
       s.pc = PC_LS_FG;
       c.sync();
       // fall through to the first line-search step at the current f, g
     }
     if (s.pc == PC_LS_FG) {
+      c.mark(PH_LNSRCH);
       const int r = lnsrch_step(c, w);
       c.sync();
       const int info = r < 0 ? r : 0;
@@ -1374,6 +1458,7 @@ __host__ __device__ inline int machine(C& c, View& w) {
       s.updatd = 1;
       s.iupdat += 1;
       c.sync();
+      c.mark(PH_UPDATE);
       matupd(c, w, rr, dr);
       const int info = formt(c, w);
       if (info) refresh(s);

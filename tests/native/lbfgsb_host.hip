@@ -8,21 +8,7 @@
 #include "../../botorch_amd/csrc/lbfgsb_core.h"
 
 namespace {
-struct HostLanes {
-  static constexpr int W = 1;
-  int lane() const { return 0; }
-  double sum(double v) const { return v; }
-  double max(double v) const { return v; }
-  double min(double v) const { return v; }
-  int sum_int(int v) const { return v; }
-  int min_int(int v) const { return v; }
-  int any(int v) const { return v != 0; }
-  int ballot_rank(int flag, int& total) const {
-    total = flag ? 1 : 0;
-    return 0;
-  }
-  void sync() const {}
-};
+using HostLanes = bta::lbfgsb::SerialLanes;
 
 struct Problem {
   bta::lbfgsb::Layout L;
