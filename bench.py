@@ -256,6 +256,66 @@ def bench_batched(steps=20, warmup=3):
             "restart_evals_per_s": nb / t_batch}
 
 
+def bench_optimize(acq, X, maxiter=50, host_iters=3):
+    """gen_candidates_scipy on the headline acquisition from the headline's
+    restarts: the device L-BFGS-B (all restarts' optimiser state on the GPU, one
+    optimiser kernel per round) for maxiter iterations, and the host loop of
+    SciPy's setulb per restart (the reference's optimiser) for host_iters
+    iterations; both per round of one value+grad step for all restarts."""
+    from botorch_amd.optim import fmin_l_bfgs_b_device, gen_candidates_scipy
+    import numpy as np
+    b, q, d = X.shape
+
+    def evaluate(x_eval):
+        return acq._launch(x_eval.view(b, q, d), True)
+
+    bounds = (np.zeros(q * d), np.ones(q * d))
+    x0 = X.reshape(b, -1)
+    fmin_l_bfgs_b_device(evaluate, x0, bounds, fscale=-1.0, maxiter=3)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    x, f, info = fmin_l_bfgs_b_device(evaluate, x0, bounds, fscale=-1.0, maxiter=maxiter)
+    torch.cuda.synchronize()
+    t_dev = time.perf_counter() - t
+    rounds = max(i["nfev"] for i in info)
+    t = time.perf_counter()
+    fmin_l_bfgs_b_device(evaluate, x0, bounds, fscale=-1.0, maxiter=maxiter, use_graph=True)
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t
+    host_rounds = [0]
+
+    class Counting:  # the same acquisition, counting the host loop's rounds
+        def __init__(self, a):
+            self.a = a
+            self.X_baseline = a.X_baseline
+
+        def value_and_grad(self, Xq):
+            host_rounds[0] += 1
+            return self.a.value_and_grad(Xq)
+
+        def __call__(self, Xq):
+            return self.a(Xq)
+
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    gen_candidates_scipy(X, Counting(acq), 0.0, 1.0,
+                         options={"maxiter": host_iters, "device_lbfgsb": False})
+    torch.cuda.synchronize()
+    t_host = time.perf_counter() - t
+    log(f"optimize: device L-BFGS-B {t_dev * 1e3:.1f} ms for {rounds} rounds "
+        f"({t_dev / rounds * 1e3:.3f} ms/round, graph {t_graph / rounds * 1e3:.3f}); "
+        f"host SciPy loop {t_host / host_rounds[0] * 1e3:.2f} ms/round")
+    return {"workload": f"gen_candidates_scipy L-BFGS-B from the {b} headline restarts "
+                        f"(q={q}, d={d}, n_vars={q * d}), maxiter={maxiter}",
+            "device_ms": t_dev * 1e3, "rounds": rounds,
+            "device_ms_per_round": t_dev / rounds * 1e3,
+            "device_graph_ms_per_round": t_graph / rounds * 1e3,
+            "mean_iterations": float(np.mean([i["nit"] for i in info])),
+            "host_scipy_ms_per_round": t_host / host_rounds[0] * 1e3,
+            "host_rounds_timed": host_rounds[0],
+            "speedup_device_vs_host_loop": (t_host / host_rounds[0]) / (t_dev / rounds)}
+
+
 def build_mll(n=4096, d=16):
     from botorch_amd import workloads
     w = workloads.gp_mll(n=n, d=d)
@@ -275,6 +335,7 @@ def main():
     ap.add_argument("--no-mll", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-optim", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="launch only a copy and the forward contraction (counter passes)")
     args = ap.parse_args()
@@ -416,6 +477,11 @@ def main():
     if not args.no_batch:
         log("batched uneven models")
         batched = bench_batched()
+    # ---------------- optimiser rounds -------------------------------------------------
+    optim = None
+    if not args.no_optim:
+        log("optimize rounds")
+        optim = bench_optimize(acq_m, X)
     # ---------------- CPU baseline --------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -442,6 +508,7 @@ def main():
         "step_tflops_mixed": step_flops / t_mixed / 1e12,
         "accuracy_max_rel_err_vs_fp64_oracle": acc,
         "roofline": roof, "cpu_baseline": cpu, "mll": mll, "batched_uneven": batched,
+        "optimize_acqf_rounds": optim,
         "autotuned_plans": plans,
     }
     print(json.dumps(res))
