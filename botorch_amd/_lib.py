@@ -44,7 +44,6 @@ GEMM_B_KN = 32
 GEMM_F32 = 64
 GEMM_A_KM = 128
 GEMM_ACC64 = 256
-GEMM_F16X3 = 512
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32

@@ -275,14 +275,12 @@ extern "C" int bta_gemm(int32_t flags, int64_t M, int64_t N, int64_t K, double a
     bta::set_error("bta_gemm: invalid pointers or sizes");
     return BTA_ERR_INVALID;
   }
-  if (flags & BTA_GEMM_F16X3) {
-    if (!(flags & BTA_GEMM_F32) || batch != 1 || C0 || beta != 0.0) {
-      bta::set_error("bta_gemm: F16X3 needs F32 operands, batch 1 and no C0");
-      return BTA_ERR_INVALID;
-    }
-    return bta::gemm_x3(flags & ~(BTA_GEMM_F16X3 | BTA_GEMM_F32 | BTA_GEMM_ACC64), M, N, K,
-                        alpha, (const float*)A, lda, nullptr, 1.0f, (const float*)B, ldb, 1.0f,
-                        (float*)C, ldc, (hipStream_t)stream);
+  const int32_t known = BTA_GEMM_A_LOWER | BTA_GEMM_A_UPPER | BTA_GEMM_B_LOWER |
+                        BTA_GEMM_B_UPPER | BTA_GEMM_C_LOWER | BTA_GEMM_B_KN | BTA_GEMM_F32 |
+                        BTA_GEMM_A_KM | BTA_GEMM_ACC64;
+  if (flags & ~known) {
+    bta::set_error("bta_gemm: unknown flag bits 0x%x", flags & ~known);
+    return BTA_ERR_INVALID;
   }
   return bta::launch_gemm(g, (int)batch, (hipStream_t)stream);
 }

@@ -52,10 +52,6 @@ int launch_baseline_best_f(const double* ka, double y_mean, double y_std, double
                            const double* Lb, const double* Zb, int n_b, int S, double* best_f,
                            hipStream_t s);
 
-int gemm_x3(int flags, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
-            int64_t lda, const float* a_row_scale, float a_scale, const float* B, int64_t ldb,
-            float b_scale, float* C, int64_t ldc, hipStream_t s);
-
 // Split-FP16 GEMM on pre-split operands (gemm_h2.hip).
 struct H2Args {
   const _Float16 *Ah, *Al;  // M x K planes (row stride lda halfs)

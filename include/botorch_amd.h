@@ -61,9 +61,7 @@ enum {
   BTA_GEMM_B_KN = 32,      /* B stored K x N row-major (else N x K row-major) */
   BTA_GEMM_F32 = 64,       /* A, B, C, C0 are float (else double) */
   BTA_GEMM_A_KM = 128,     /* A stored K x M row-major (else M x K row-major) */
-  BTA_GEMM_ACC64 = 256,    /* float GEMM: add each 256-wide K segment into FP64 sums */
-  BTA_GEMM_F16X3 = 512     /* float GEMM on split-FP16 MFMA (M%128, N%64, batch 1, no C0;
-                              operands must lie within the FP16 range) */
+  BTA_GEMM_ACC64 = 256     /* float GEMM: add each 256-wide K segment into FP64 sums */
 };
 
 typedef void* bta_stream_t; /* hipStream_t */

@@ -18,35 +18,6 @@ def _say(msg):
     print(f"[case] {msg}", flush=True)
 
 
-@pytest.mark.parametrize("flags,shape", [
-    (0, (128, 128, 256)),
-    (_lib.GEMM_B_LOWER, (256, 512, 512)),
-    (_lib.GEMM_B_KN | _lib.GEMM_B_UPPER, (256, 512, 512)),
-    (_lib.GEMM_B_LOWER, (2048, 2048, 2048)),
-])
-def test_gemm_f16x3_matches_fp64(flags, shape):
-    _say(f"gemm f16x3 flags={flags} shape={shape}")
-    M, N, K = shape
-    g = torch.Generator().manual_seed(M + N + K + flags)
-    A = (torch.rand(M, K, generator=g, dtype=torch.float64) - 0.3).float().double()
-    opB = (torch.rand(N, K, generator=g, dtype=torch.float64) - 0.5).float().double()
-    if flags & _lib.GEMM_B_LOWER:
-        opB = torch.tril(opB)
-    if flags & _lib.GEMM_B_UPPER:
-        opB = torch.triu(opB)
-    ref = A @ opB.t()
-    Bstore = opB.t().contiguous() if flags & _lib.GEMM_B_KN else opB.contiguous()
-    Ad, Bd = A.float().to(DEV), Bstore.float().to(DEV)
-    C = torch.empty(M, N, dtype=torch.float32, device=DEV)
-    ldb = N if flags & _lib.GEMM_B_KN else K
-    _lib.check(_lib.lib().bta_gemm(flags | _lib.GEMM_F32 | _lib.GEMM_F16X3, M, N, K, 1.0,
-                                   _lib.ptr(Ad), K, _lib.ptr(Bd), ldb, 0.0, None, 0, _lib.ptr(C),
-                                   N, 1, 0, 0, 0, 0, _lib.stream()))
-    err = ((C.double().cpu() - ref).abs().max() / ref.abs().max()).item()
-    # operands carried to 22 bits: error of the order of FP32 rounding
-    assert err < 2e-6, err
-
-
 def _split(X, rows_pad=None, cols_pad=None, scale=1.0, transpose=False):
     """FP16 hi/lo planes of scale * X through bta_split_f16 (X double on the device)."""
     rows, cols = X.shape
