@@ -727,8 +727,10 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
                          s, A, lda, n_pad, jit);
       BTA_LAUNCH_CHECK("add_diag_kernel");
     }
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, info, 2, 0);
-    BTA_LAUNCH_CHECK("fill_int_kernel");
+    if (attempt > 0 || !(opts & POTRF_INFO_ZEROED)) {
+      hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, info, 2, 0);
+      BTA_LAUNCH_CHECK("fill_int_kernel");
+    }
     for (int j = 0; j < NB; ++j) {
       const int rc = launch_step(A, lda, info, (opts & POTRF_BAND_ZERO) ? Linv : nullptr, ldinv, j,
                                  NB, num_cus(), s);

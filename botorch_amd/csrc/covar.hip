@@ -87,28 +87,42 @@ __global__ __launch_bounds__(256) void covar_kernel(const double* __restrict__ X
 // Symmetric case for the factorisation: K = s k(Xs, Xs) + diag_add I on and
 // below the diagonal only (the Cholesky reads nothing above it), identity on
 // the padding, kernel kind fixed at compile time.  Blocks wholly above the
-// diagonal exit at once.
+// diagonal exit at once.  The block's ROWS row points and its 256 column
+// points are staged in LDS by coalesced loads (a column point's coordinates
+// are contiguous, so lane-per-column global loads would touch a cache line per
+// lane); the odd pitch MD + 1 keeps the per-lane column reads conflict-free.
+// When zero2 is set, the first block also zeroes those two ints (the status
+// pair of the factorisation that follows).
 template <int KIND, int MD, int ROWS>
 __global__ __launch_bounds__(256) void covar_sym_lower_kernel(const double* __restrict__ Xs,
                                                               int64_t n, int d, double s,
                                                               double diag_add,
                                                               double* __restrict__ K,
-                                                              int64_t ldk, int64_t n_pad) {
+                                                              int64_t ldk, int64_t n_pad,
+                                                              int* zero2) {
   __shared__ double xr[ROWS * MD];
+  __shared__ double xc[256 * (MD + 1)];
+  if (zero2 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) zero2[threadIdx.x] = 0;
   const int64_t r0 = (int64_t)blockIdx.y * ROWS;
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if ((int64_t)blockIdx.x * 256 > r0 + ROWS - 1) return;  // block-uniform
+  const int64_t c0 = (int64_t)blockIdx.x * 256;
+  const int64_t c = c0 + threadIdx.x;
+  if (c0 > r0 + ROWS - 1) return;  // block-uniform
   for (int i = threadIdx.x; i < ROWS * MD; i += 256) {
     const int64_t r = r0 + i / MD;
     const int j = i % MD;
     xr[i] = (r < n && j < d) ? Xs[r * d + j] : 0.0;
+  }
+  for (int i = threadIdx.x; i < 256 * MD; i += 256) {
+    const int64_t cc = c0 + i / MD;
+    const int j = i % MD;
+    xc[(i / MD) * (MD + 1) + j] = (cc < n && j < d) ? Xs[cc * d + j] : 0.0;
   }
   __syncthreads();
   if (c >= n_pad) return;
   double t[MD];
   const bool cvalid = c < n;
 #pragma unroll
-  for (int j = 0; j < MD; ++j) t[j] = (j < d && cvalid) ? Xs[c * d + j] : 0.0;
+  for (int j = 0; j < MD; ++j) t[j] = xc[threadIdx.x * (MD + 1) + j];
 #pragma unroll 2
   for (int rr = 0; rr < ROWS; ++rr) {
     const int64_t r = r0 + rr;
@@ -132,7 +146,7 @@ __global__ __launch_bounds__(256) void covar_sym_lower_kernel(const double* __re
 }
 
 int covar_sym_lower(int kind, const double* Xs, int64_t n, int d, double s, double diag_add,
-                    double* K, int64_t ldk, int64_t n_pad, hipStream_t st) {
+                    double* K, int64_t ldk, int64_t n_pad, hipStream_t st, int* zero2) {
   if (d < 1 || d > MAXD || n_pad < n || ldk < n_pad || n < 0) {
     set_error("covar_sym_lower: bad sizes (d=%d n=%ld)", d, (long)n);
     return BTA_ERR_INVALID;
@@ -148,17 +162,17 @@ int covar_sym_lower(int kind, const double* Xs, int64_t n, int d, double s, doub
   if (d <= 16) {
     if (m52)
       hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_MATERN52, 16, ROWS>), grid,
-                         dim3(256), 0, st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+                         dim3(256), 0, st, Xs, n, d, s, diag_add, K, ldk, n_pad, zero2);
     else
       hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_RBF, 16, ROWS>), grid, dim3(256), 0,
-                         st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+                         st, Xs, n, d, s, diag_add, K, ldk, n_pad, zero2);
   } else {
     if (m52)
       hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_MATERN52, MAXD, ROWS>), grid,
-                         dim3(256), 0, st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+                         dim3(256), 0, st, Xs, n, d, s, diag_add, K, ldk, n_pad, zero2);
     else
       hipLaunchKernelGGL((covar_sym_lower_kernel<BTA_KERNEL_RBF, MAXD, ROWS>), grid, dim3(256),
-                         0, st, Xs, n, d, s, diag_add, K, ldk, n_pad);
+                         0, st, Xs, n, d, s, diag_add, K, ldk, n_pad, zero2);
   }
   BTA_LAUNCH_CHECK("covar_sym_lower_kernel");
   return BTA_OK;

@@ -178,8 +178,9 @@ static double d_cost(const GemmArgs& g, int batch, int BN, int W, int nsplit,
   }
   double t = 0.0;
   for (auto& v : per_cu) t = std::max(t, cu_time(v));
-  // zeroing C before a split launch: M N batch doubles at about 3 TB/s
-  if (nsplit > 0) t += 2.0 + (double)g.M * g.N * batch * 8.0 / 3e12 / 0.8e-6;
+  // zeroing C before a split launch: one more launch (about 5 us measured for
+  // the small ones) plus M N batch doubles at about 3 TB/s
+  if (nsplit > 0) t += 6.5 + (double)g.M * g.N * batch * 8.0 / 3e12 / 0.8e-6;
   return t;
 }
 

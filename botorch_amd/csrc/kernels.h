@@ -13,7 +13,8 @@ int covar(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2
 // Lower triangle (diagonal included) of s k(Xs, Xs) + diag_add I, identity
 // on the padding rows/columns; nothing above the diagonal is written.
 int covar_sym_lower(int kind, const double* Xs, int64_t n, int d, double s, double diag_add,
-                    double* K, int64_t ldk, int64_t n_pad, hipStream_t st);
+                    double* K, int64_t ldk, int64_t n_pad, hipStream_t st,
+                    int* zero2 = nullptr);
 int covar_grad(int kind, const double* X1s, int64_t n1, const double* X2s, int64_t n2, int d,
                double s, const void* G, bool g_f32, int64_t ldg, const double* extra,
                const double* ls, double* dX1, hipStream_t st, const double* dext = nullptr,
@@ -34,8 +35,10 @@ size_t potrf_ws_bytes(int64_t n_pad);
 // potrf_status(ws, n_pad) once the stream has passed the factorisation
 // ({0, 0} = factorised; else {failing column + 1, NaN seen}); the caller checks
 // it and discards what it enqueued after a failure.  trtri_lower may reuse the
-// same workspace: its scratch ends before the pair.
-constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2, POTRF_BAND_ZERO = 4, POTRF_ASYNC = 8;
+// same workspace: its scratch ends before the pair.  POTRF_INFO_ZEROED: the
+// caller has already zeroed that pair on the stream, so no fill is launched.
+constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2, POTRF_BAND_ZERO = 4, POTRF_ASYNC = 8,
+              POTRF_INFO_ZEROED = 16;
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
           double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
           int opts = 0);

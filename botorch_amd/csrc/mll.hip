@@ -178,10 +178,12 @@ __global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(Trac
         red[e] + red[(MD + 2) + e] + red[2 * (MD + 2) + e] + red[3 * (MD + 2) + e];
 }
 
-// t = X r for the lower-triangular X (one wave per row, double2 loads), and
+// t = X r for the lower-triangular X (one wave per row, double2 loads) with the
+// residual r_k = y_k - c (0 in the padding) formed on the fly, and
 // lgd_i = log L_ii on the side
 __global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restrict__ X, int64_t ld,
-                                                         const double* __restrict__ r, int64_t n,
+                                                         const double* __restrict__ y, double c,
+                                                         int64_t ny, int64_t n,
                                                          double* __restrict__ t,
                                                          const double* __restrict__ L,
                                                          double* __restrict__ lgd) {
@@ -192,8 +194,9 @@ __global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restric
   double acc = 0.0;
   for (int64_t k = 2 * lane; k <= i; k += 128) {
     const v2d x = *(const v2d*)(row + k);
-    acc = fma(x.x, r[k], acc);
-    if (k + 1 <= i) acc = fma(x.y, r[k + 1], acc);
+    const double r0 = k < ny ? y[k] - c : 0.0;
+    acc = fma(x.x, r0, acc);
+    if (k + 1 <= i) acc = fma(x.y, k + 1 < ny ? y[k + 1] - c : 0.0, acc);
   }
   acc = wave_sum(acc);
   if (lane == 0) {
@@ -249,8 +252,10 @@ template <int NP2>
 __global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const double* lgd,
                                                          const double* alpha, int64_t n,
                                                          const double* partial, int W,
-                                                         double* out) {
+                                                         double* out, const int* stat,
+                                                         int* stat_out) {
   __shared__ double red[8 * 40];
+  if (threadIdx.x < 2) stat_out[threadIdx.x] = stat[threadIdx.x];
   __shared__ double red3[4 * 3];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double q = 0.0, lg = 0.0, sa = 0.0;
@@ -306,15 +311,11 @@ __global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const 
     out[3 + e] = red[e] + red[40 + e] + red[80 + e] + red[120 + e];
 }
 
-__global__ void mll_residual_kernel(const double* y, int64_t n, double c, double* r, int64_t n_pad) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_pad) r[i] = i < n ? y[i] - c : 0.0;
-}
 
 struct MllWs {
   int64_t n_pad;
   int W, MD;
-  size_t ls, center, xs, L, Linv, fact, r, t, lgd, alpha, part, partial, out, total;
+  size_t ls, center, xs, L, Linv, fact, t, lgd, alpha, part, partial, out, total;
   MllWs(int64_t n, int d) {
     n_pad = round_up(n, 128);
     MD = d <= 16 ? 16 : 32;
@@ -326,7 +327,6 @@ struct MllWs {
     L = p.take((size_t)n_pad * n_pad * 8);
     Linv = p.take((size_t)n_pad * n_pad * 8);
     fact = p.take(std::max(potrf_ws_bytes(n_pad), trtri_ws_bytes(n_pad)));
-    r = p.take((size_t)n_pad * 8);
     t = p.take((size_t)n_pad * 8);
     lgd = p.take((size_t)n_pad * 8);
     alpha = p.take((size_t)n_pad * 8);
@@ -355,7 +355,6 @@ static int enqueue_mll(const MllWs& w, int kind, int64_t n, int d, double s, dou
   double* xs = (double*)(base + w.xs);
   double* L = (double*)(base + w.L);
   double* Linv = (double*)(base + w.Linv);
-  double* r = (double*)(base + w.r);
   double* t = (double*)(base + w.t);
   double* lgd = (double*)(base + w.lgd);
   double* alpha = (double*)(base + w.alpha);
@@ -363,18 +362,17 @@ static int enqueue_mll(const MllWs& w, int kind, int64_t n, int d, double s, dou
   double* partial = (double*)(base + w.partial);
   double* out = (double*)(base + w.out);
   const int64_t np = w.n_pad;
-  int rc = covar_sym_lower(kind, xs, n, d, s, diag, L, np, np, st);
+  int rc = covar_sym_lower(kind, xs, n, d, s, diag, L, np, np, st,
+                           const_cast<int*>(potrf_status(fact, np)));
   if (rc) return rc;
   rc = potrf(L, np, np, Linv, np, 0, 1e-8, nullptr, fact, fact_bytes, st,
-             POTRF_NO_SAVE | POTRF_KEEP_UPPER | POTRF_BAND_ZERO | POTRF_ASYNC);
+             POTRF_NO_SAVE | POTRF_KEEP_UPPER | POTRF_BAND_ZERO | POTRF_ASYNC |
+                 POTRF_INFO_ZEROED);
   if (rc) return rc;
   rc = trtri_lower(L, np, Linv, np, np, fact, fact_bytes, st, TRTRI_UPPER_ZERO);
   if (rc) return rc;
-  hipLaunchKernelGGL(mll_residual_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, y, n,
-                     c, r, np);
-  BTA_LAUNCH_CHECK("mll_residual_kernel");
   hipLaunchKernelGGL(trmv_lower_kernel, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, Linv,
-                     np, r, np, t, L, lgd);
+                     np, y, c, n, np, t, L, lgd);
   BTA_LAUNCH_CHECK("trmv_lower_kernel");
   {
     const unsigned gx = (unsigned)std::min<int64_t>((np + 255) / 256, 16);
@@ -404,15 +402,16 @@ static int enqueue_mll(const MllWs& w, int kind, int64_t n, int d, double s, dou
   else
     launch_trace<32>(kind, ta, W, st);
   BTA_LAUNCH_CHECK("mll_trace_kernel");
+  // the factorisation's status pair goes next to the sums (read back with them)
+  const int* stat = potrf_status(fact, np);
+  int* stat_out = (int*)(out + 3 + w.MD + 2);
   if (w.MD == 16)
     hipLaunchKernelGGL(mll_finish_kernel<18>, dim3(1), dim3(256), 0, st, t, lgd, alpha, n, partial,
-                       W, out);
+                       W, out, stat, stat_out);
   else
     hipLaunchKernelGGL(mll_finish_kernel<34>, dim3(1), dim3(256), 0, st, t, lgd, alpha, n, partial,
-                       W, out);
+                       W, out, stat, stat_out);
   BTA_LAUNCH_CHECK("mll_finish_kernel");
-  BTA_HIP(hipMemcpyAsync(out + 3 + w.MD + 2, potrf_status(fact, np), 2 * sizeof(int),
-                         hipMemcpyDeviceToDevice, st));
   return BTA_OK;
 }
 
