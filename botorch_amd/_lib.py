@@ -111,6 +111,8 @@ SIGNATURES = {
     "bta_potrf_workspace_size": (_sz, [_i64]),
     "bta_potrf": (ctypes.c_int, [_p, _i64, _i64, _p, _i64, _i32, _f64,
                                  ctypes.POINTER(_f64), _p, _sz, _p]),
+    "bta_potrf_inverse": (ctypes.c_int, [_p, _i64, _i64, _p, _i64, _i32, _f64,
+                                         ctypes.POINTER(_f64), _p, _sz, _p]),
     "bta_trtri_lower": (ctypes.c_int, [_p, _i64, _p, _i64, _i64, _p, _sz, _p]),
     "bta_chol_small_batched": (ctypes.c_int, [_p, _i32, _i64, _i64, _i64, _i32, _f64, _p, _p]),
     "bta_gp_cache_workspace_size": (_sz, [_i64]),

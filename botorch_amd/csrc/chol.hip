@@ -22,9 +22,14 @@
 #include "bta_common.h"
 #include "gemm.h"
 #include "gemm_d.h"
+#include "inv_plan.h"
 #include "kernels.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
+#include <vector>
 
 namespace bta {
 
@@ -381,6 +386,8 @@ struct StepArgs {
   double* Linv;  // when set: the diagonal workgroup also writes L_jj^-1 into
   int64_t ldinv;  // Linv's diagonal block and zeroes the block right of it
   int j, NB, nstrip;
+  const int4* side;  // this step's inverse side tasks (POTRF_INVERT): nside of
+  int nside, npart;  // them, the first npart partial tasks, then finish tasks
 };
 
 constexpr int VP = 17;  // pitch of the 16 x 16 diagonal inverses (64 x 16 array)
@@ -506,52 +513,171 @@ __device__ __forceinline__ void potrf_strip(const StepArgs& a, unsigned char* sm
   for (int t = 0; t < 16; ++t) Abj[(int64_t)(r0 + t) * lda + lane] = Bf[(r0 + t) * LP + lane];
 }
 
-// Rank-64 update of the trailing lower matrix on 128 x 64 tiles (tile row tm
-// holds the 2 tm + 2 tiles tn <= 2 tm + 1).  Every tile costs the same
-// (K = 64), so workers take tiles in a plain stride.  The wave's part of C is
-// loaded before the products: its latency overlaps the first chunks of the
-// main loop instead of following it.
-__device__ __forceinline__ void potrf_bulk(const StepArgs& a, unsigned char* smem) {
+// Rank-64 update of one trailing lower 128 x 64 tile (tile row tm holds the
+// 2 tm + 2 tiles tn <= 2 tm + 1; rem = rows of the trailing matrix).  The
+// wave's part of C is loaded before the products: its latency overlaps the
+// first chunks of the main loop instead of following it.
+__device__ __forceinline__ void potrf_bulk_tile(const StepArgs& a, int t, int rem,
+                                                unsigned char* smem) {
   const int j = a.j;
-  const int rem = (a.NB - j - 1) * LEAF;
   const int64_t lda = a.lda;
   const double* P = a.A + (int64_t)(j + 1) * LEAF * lda + (int64_t)(j - 1) * LEAF;
   double* Cm = a.A + (int64_t)(j + 1) * LEAF * lda + (int64_t)(j + 1) * LEAF;
-  const int MT = (rem + D_BM - 1) / D_BM, total = MT * (MT + 1);
-  const int W = gridDim.x - a.nstrip, w = blockIdx.x - a.nstrip;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
-  for (int t = w; t < total; t += W) {
-    int tm = 0;
-    while ((tm + 1) * (tm + 2) <= t) ++tm;  // tile rows before tm hold tm (tm + 1) tiles
-    const int tn = t - tm * (tm + 1);
-    double cv[4][2][4];
+  int tm = 0;
+  while ((tm + 1) * (tm + 2) <= t) ++tm;  // tile rows before tm hold tm (tm + 1) tiles
+  const int tn = t - tm * (tm + 1);
+  double cv[4][2][4];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+  for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
+    for (int q = 0; q < 4; ++q) {
+      const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-          const int col = tn * 64 + wx * 32 + ni * 16 + l16;
-          cv[mi][ni][q] = (row < rem && col <= row) ? Cm[(int64_t)row * lda + col] : 0.0;
-        }
+      for (int ni = 0; ni < 2; ++ni) {
+        const int col = tn * 64 + wx * 32 + ni * 16 + l16;
+        cv[mi][ni][q] = (row < rem && col <= row) ? Cm[(int64_t)row * lda + col] : 0.0;
       }
-    v4d acc[4][2];
-    d_mainloop<64, false, false>(P, lda, P, lda, tm * D_BM, tn * 64, rem, rem, 0, LEAF, smem,
-                                 acc);
+    }
+  v4d acc[4][2];
+  d_mainloop<64, false, false>(P, lda, P, lda, tm * D_BM, tn * 64, rem, rem, 0, LEAF, smem, acc);
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+  for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
+    for (int q = 0; q < 4; ++q) {
+      const int row = tm * D_BM + wy * 64 + mi * 16 + Mfma<double>::row(gq, q);
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-          const int col = tn * 64 + wx * 32 + ni * 16 + l16;
-          if (row < rem && col <= row) Cm[(int64_t)row * lda + col] = cv[mi][ni][q] - acc[mi][ni][q];
-        }
+      for (int ni = 0; ni < 2; ++ni) {
+        const int col = tn * 64 + wx * 32 + ni * 16 + l16;
+        if (row < rem && col <= row) Cm[(int64_t)row * lda + col] = cv[mi][ni][q] - acc[mi][ni][q];
       }
+    }
+}
+
+// Inverse side tasks (POTRF_INVERT).  X = L^-1 is built row block by row
+// block alongside the factorisation:
+//   X_rc = -X_rr S_rc,  S_rc = sum_{k=c}^{r-1} L_rk X_kc   (c < r)
+// with X_rr = L_rr^-1 written by step r's diagonal workgroup.  The sums of the
+// row pair (2q, 2q+1) over k < 2q - 1 are accumulated in place in X's own
+// storage, in K chunks of 128 x 64 tiles (partial tasks) that a host-built
+// plan places in the steps whose launches start after the chunk's operands
+// are final: L_rk final after step k, X row block k after step k + 2.  The
+// finish task of row r (step r + 2) adds the remaining terms k in
+// [max(c, 2q - 1), r) and multiplies by -X_rr.  Every read and write of a
+// task is ordered against its producers by launch order alone.
+//
+// Partial task: x = 0 | first << 4, y = q, z = c, w = k0 | k1 << 16 (leaves).
+__device__ __forceinline__ void inv_partial(const StepArgs& a, int4 t, unsigned char* smem) {
+  const int q = t.y, c = t.z, k0 = t.w & 0xffff, k1 = t.w >> 16;
+  const bool first = (t.x >> 4) & 1;
+  const int np = a.NB * LEAF;
+  double* X = a.Linv;
+  const int64_t ldx = a.ldinv;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wy = wv >> 1, wx = wv & 1, l16 = lane & 15, gq = lane >> 4;
+  const int row0 = q * D_BM, col0 = c * LEAF;
+  double cv[4][2][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int row = row0 + wy * 64 + mi * 16 + Mfma<double>::row(gq, qq);
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) {
+        const int col = col0 + wx * 32 + ni * 16 + l16;
+        cv[mi][ni][qq] = (!first && row < np) ? X[(int64_t)row * ldx + col] : 0.0;
+      }
+    }
+  v4d acc[4][2];
+  d_mainloop<64, false, true>(a.A, a.lda, X, ldx, row0, col0, np, np, k0 * LEAF, k1 * LEAF, smem,
+                              acc);
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int row = row0 + wy * 64 + mi * 16 + Mfma<double>::row(gq, qq);
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) {
+        const int col = col0 + wx * 32 + ni * 16 + l16;
+        if (row < np) X[(int64_t)row * ldx + col] = cv[mi][ni][qq] + acc[mi][ni][qq];
+      }
+    }
+}
+
+// Finish task of X_rc: x = 1, y = r, z = c.  Wave w holds rows [16w, 16w + 16)
+// of the 64 x 64 block, its four 16-column blocks in turn.
+__device__ __forceinline__ void inv_finish(const StepArgs& a, int4 t, unsigned char* smem) {
+  const int r = t.y, c = t.z, q = r >> 1;
+  const bool accumulated = q >= 1 && c <= 2 * q - 2;  // partial tasks wrote S_rc's first terms
+  double* X = a.Linv;
+  const int64_t ldx = a.ldinv, lda = a.lda;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
+  double* La = (double*)smem;  // L_rk, then X_rr
+  double* Xb = La + LEAF * LP;  // X_kc, then S_rc
+  double* Xrc = X + (int64_t)r * LEAF * ldx + (int64_t)c * LEAF;
+  v4d acc[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+      acc[cb][qq] = accumulated ? Xrc[(int64_t)(16 * wv + g + 4 * qq) * ldx + 16 * cb + l16] : 0.0;
+  // acc[cb] += sgn * La[16 wv.., 16 s..] Xb[16 s.., 16 cb..] for s < s_end, one
+  // 16-column slab of k at a time (the four accumulators share the A operand)
+  auto rows_times = [&](int s_end, double sgn, v4d(&o)[4]) {
+#pragma unroll 1
+    for (int s = 0; s < s_end; ++s)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int k = 16 * s + 4 * kk + g;
+        const double av = sgn * La[(16 * wv + l16) * LP + k];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+          o[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Xb[k * LP + 16 * cb + l16], o[cb], 0, 0,
+                                                      0);
+      }
+  };
+  for (int k = max(c, 2 * q - 1); k < r; ++k) {
+    stage64(a.A + (int64_t)r * LEAF * lda + (int64_t)k * LEAF, lda, La);
+    stage64(X + (int64_t)k * LEAF * ldx + (int64_t)c * LEAF, ldx, Xb);
+    __syncthreads();
+    rows_times(4, 1.0, acc);
+    __syncthreads();
   }
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) st16(Xb, 16 * wv, 16 * cb, acc[cb]);
+  stage64(X + (int64_t)r * LEAF * ldx + (int64_t)r * LEAF, ldx, La);
+  __syncthreads();
+  v4d o[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) o[cb] = (v4d)(0.0);
+  rows_times(wv + 1, -1.0, o);  // X_rr is lower triangular: its blocks s > wv are zero
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+      Xrc[(int64_t)(16 * wv + g + 4 * qq) * ldx + 16 * cb + l16] = o[cb][qq];
+  __syncthreads();  // the next task may overwrite La / Xb
+}
+
+// The non-strip workgroups of a step: the step's side tasks (when the plan
+// has any) and then the trailing update's tiles, dealt in a plain stride.
+__device__ __forceinline__ void potrf_work(const StepArgs& a, unsigned char* smem) {
+  const int j = a.j;
+  int rem = 0, ntile = 0;
+  if (j > 0 && a.NB - j - 1 > 0) {
+    rem = (a.NB - j - 1) * LEAF;
+    const int MT = (rem + D_BM - 1) / D_BM;
+    ntile = MT * (MT + 1);
+  }
+  const int total = a.nside + ntile;
+  const int W = gridDim.x - a.nstrip;
+  // task t = w, w + W, ...: the partial tasks (the plan's first npart), the
+  // finish tasks, then the tiles, each kind in a loop of its own
+  int t = blockIdx.x - a.nstrip;
+  for (; t < a.npart; t += W) inv_partial(a, a.side[t], smem);
+  for (; t < a.nside; t += W) inv_finish(a, a.side[t], smem);
+  for (; t < total; t += W) potrf_bulk_tile(a, t - a.nside, rem, smem);
 }
 
 __global__ __launch_bounds__(256, 2) void potrf_step_kernel(StepArgs a) {
@@ -563,7 +689,7 @@ __global__ __launch_bounds__(256, 2) void potrf_step_kernel(StepArgs a) {
   if ((int)blockIdx.x < a.nstrip)
     potrf_strip(a, smem);
   else
-    potrf_bulk(a, smem);
+    potrf_work(a, smem);
 }
 
 // W_jj = L_jj^-1 for every 64 x 64 diagonal block of L (one workgroup per
@@ -656,10 +782,53 @@ size_t potrf_ws_bytes(int64_t n_pad) {
   return (size_t)(n_pad * n_pad + LEAF * LEAF) * sizeof(double) + 256;
 }
 
+// Device copy of the plan for (device, NB, CU count), built and uploaded on
+// first use and kept until the process ends.
+struct InvPlanDev {
+  const int4* task = nullptr;
+  std::vector<int> off, units, npart;
+};
+static int inv_plan(int NB, int ncu, const InvPlanDev** out) {
+  static std::mutex& mu = *new std::mutex;  // never destroyed
+  static std::map<std::array<int, 3>, InvPlanDev>& cache =
+      *new std::map<std::array<int, 3>, InvPlanDev>;
+  int dev = 0;
+  BTA_HIP(hipGetDevice(&dev));
+  const std::array<int, 3> key = {dev, NB, ncu};
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    const invplan::Plan P = invplan::build(NB, ncu);
+    if (P.off.empty()) {
+      set_error("potrf: inverse plan for %d block rows missed a deadline", NB);
+      return BTA_ERR_INVALID;
+    }
+    InvPlanDev D;
+    D.off = P.off;
+    D.units = P.units;
+    D.npart = P.npart;
+    if (!P.task.empty()) {
+      static_assert(sizeof(invplan::Task) == sizeof(int4), "task layout");
+      void* p = nullptr;
+      BTA_HIP(hipMalloc(&p, P.task.size() * sizeof(int4)));
+      const hipError_t e = hipMemcpy(p, P.task.data(), P.task.size() * sizeof(int4),
+                                     hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, "hipMemcpy(inverse plan)");
+      }
+      D.task = (const int4*)p;
+    }
+    it = cache.emplace(key, std::move(D)).first;
+  }
+  *out = &it->second;
+  return BTA_OK;
+}
+
 // Step j of the factorisation of the NB x NB block matrix A on a stream whose
-// launches reach ncu CUs.
+// launches reach ncu CUs, with the plan's side tasks when plan is set.
 static int launch_step(double* A, int64_t lda, int* info, double* Linv, int64_t ldinv, int j,
-                       int NB, int ncu, hipStream_t s) {
+                       int NB, int ncu, const InvPlanDev* plan, hipStream_t s) {
   StepArgs sa;
   sa.A = A;
   sa.lda = lda;
@@ -668,19 +837,28 @@ static int launch_step(double* A, int64_t lda, int* info, double* Linv, int64_t 
   sa.ldinv = ldinv;
   sa.j = j;
   sa.NB = NB;
-  sa.nstrip = NB - j;
-  int bulk = 0;
-  if (j > 0 && NB - j - 1 > 0) {
+  sa.nstrip = std::max(NB - j, 0);
+  sa.side = nullptr;
+  sa.nside = sa.npart = 0;
+  int units = invplan::bulk_tiles(NB, j);
+  if (plan) {
+    sa.side = plan->task + plan->off[j];
+    sa.nside = plan->off[j + 1] - plan->off[j];
+    sa.npart = plan->npart[j];
+    units += plan->units[j];
+  }
+  const int tasks = invplan::bulk_tiles(NB, j) + sa.nside;
+  int work = 0;
+  if (tasks > 0) {
     // The strip workgroups (dispatched first, one per CU) are the step's
-    // critical path once the update is small: if at most two rounds of tiles
+    // critical path once the update is small: if at most two rounds of work
     // fit on the CUs they leave free, keep every strip workgroup alone on its
     // CU; otherwise fill the second slot of every CU.
-    const int MT = (NB - j - 1 + 1) / 2;  // 128-row tile rows over the trailing blocks
-    const int tiles = MT * (MT + 1);
     const int w1 = std::max(ncu - sa.nstrip, 1), w2 = std::max(2 * ncu - sa.nstrip, 1);
-    bulk = std::min(tiles, (tiles + w1 - 1) / w1 <= 2 ? w1 : w2);
+    work = std::min(tasks, (units + w1 - 1) / w1 <= 2 ? w1 : w2);
   }
-  hipLaunchKernelGGL(potrf_step_kernel, dim3((unsigned)(sa.nstrip + bulk)), dim3(256), 0, s, sa);
+  if (sa.nstrip + work == 0) return BTA_OK;
+  hipLaunchKernelGGL(potrf_step_kernel, dim3((unsigned)(sa.nstrip + work)), dim3(256), 0, s, sa);
   BTA_LAUNCH_CHECK("potrf_step_kernel");
   return BTA_OK;
 }
@@ -706,11 +884,25 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
     set_error("potrf: workspace too small (%zu < %zu)", ws_bytes, potrf_ws_bytes(n_pad));
     return BTA_ERR_WORKSPACE;
   }
+  const bool invert = (opts & POTRF_INVERT) != 0;
+  if (invert && (!Linv || ((uintptr_t)Linv & 15) || (ldinv & 1))) {
+    set_error("potrf: the inverse needs Linv 16-byte aligned with an even leading dimension");
+    return BTA_ERR_INVALID;
+  }
   double* saved = (double*)ws;
   double* wblk = saved + n_pad * n_pad;
   int* info = (int*)(wblk + LEAF * LEAF);
   if (jitter_used) *jitter_used = 0.0;
   const int NB = (int)(n_pad / LEAF);
+  const int ncu = num_cus();
+  const InvPlanDev* plan = nullptr;
+  if (invert) {
+    const int rc = inv_plan(NB, ncu, &plan);
+    if (rc) return rc;
+  }
+  // with the inverse: the diagonal blocks and the zero blocks right of them
+  // are written by the steps, the rest of X by the side tasks
+  const bool band = invert || (opts & POTRF_BAND_ZERO);
   if (save) {
     dim3 g((unsigned)((n_pad + 255) / 256 > 64 ? 64 : (n_pad + 255) / 256), (unsigned)n_pad);
     hipLaunchKernelGGL(copy_lower_kernel, g, dim3(256), 0, s, A, lda, saved, n_pad, n_pad);
@@ -731,9 +923,8 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
       hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, info, 2, 0);
       BTA_LAUNCH_CHECK("fill_int_kernel");
     }
-    for (int j = 0; j < NB; ++j) {
-      const int rc = launch_step(A, lda, info, (opts & POTRF_BAND_ZERO) ? Linv : nullptr, ldinv, j,
-                                 NB, num_cus(), s);
+    for (int j = 0; j < (invert ? NB + 2 : NB); ++j) {
+      const int rc = launch_step(A, lda, info, band ? Linv : nullptr, ldinv, j, NB, ncu, plan, s);
       if (rc) return rc;
     }
     if (opts & POTRF_ASYNC) {  // the caller reads info after its own synchronisation
@@ -742,10 +933,14 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
         hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, A, lda, n_pad);
         BTA_LAUNCH_CHECK("zero_upper_kernel");
       }
-      if (Linv && !(opts & POTRF_BAND_ZERO)) {
+      if (Linv && !band) {
         hipLaunchKernelGGL(trinv_diag_kernel, dim3((unsigned)NB), dim3(256), 0, s, A, lda, Linv,
                            ldinv, n_pad);
         BTA_LAUNCH_CHECK("trinv_diag_kernel");
+      }
+      if (invert && !(opts & POTRF_BAND_ZERO)) {
+        hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, Linv, ldinv, n_pad);
+        BTA_LAUNCH_CHECK("zero_upper_kernel");
       }
       return BTA_OK;
     }
@@ -758,10 +953,14 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
         hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, A, lda, n_pad);
         BTA_LAUNCH_CHECK("zero_upper_kernel");
       }
-      if (Linv && !(opts & POTRF_BAND_ZERO)) {  // (with it, the steps wrote them)
+      if (Linv && !band) {  // (with it, the steps wrote them)
         hipLaunchKernelGGL(trinv_diag_kernel, dim3((unsigned)NB), dim3(256), 0, s, A, lda, Linv,
                            ldinv, n_pad);
         BTA_LAUNCH_CHECK("trinv_diag_kernel");
+      }
+      if (invert && !(opts & POTRF_BAND_ZERO)) {
+        hipLaunchKernelGGL(zero_upper_kernel, g, dim3(256), 0, s, Linv, ldinv, n_pad);
+        BTA_LAUNCH_CHECK("zero_upper_kernel");
       }
       if (jitter_used) *jitter_used = jit;
       return BTA_OK;
@@ -924,6 +1123,18 @@ extern "C" int bta_potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, in
   }
   return bta::potrf(A, n_pad, lda, Linv, ldinv, max_tries, jitter0, jitter_used, ws, ws_bytes,
                     (hipStream_t)stream, 0);
+}
+
+extern "C" int bta_potrf_inverse(double* A, int64_t n_pad, int64_t lda, double* Linv,
+                                 int64_t ldinv, int32_t max_tries, double jitter0,
+                                 double* jitter_used, void* ws, size_t ws_bytes,
+                                 bta_stream_t stream) {
+  if (!A || !Linv) {
+    bta::set_error("bta_potrf_inverse: null pointer");
+    return BTA_ERR_INVALID;
+  }
+  return bta::potrf(A, n_pad, lda, Linv, ldinv, max_tries, jitter0, jitter_used, ws, ws_bytes,
+                    (hipStream_t)stream, bta::POTRF_INVERT);
 }
 
 extern "C" int bta_trtri_lower(const double* L, int64_t ldl, double* Linv, int64_t ldinv,

@@ -37,8 +37,12 @@ size_t potrf_ws_bytes(int64_t n_pad);
 // it and discards what it enqueued after a failure.  trtri_lower may reuse the
 // same workspace: its scratch ends before the pair.  POTRF_INFO_ZEROED: the
 // caller has already zeroed that pair on the stream, so no fill is launched.
+// POTRF_INVERT (with Linv, 16-byte aligned with an even ldinv): Linv receives
+// all of L^-1, built by side tasks of the factorisation steps and two more
+// launches after them (no trtri_lower); its strict upper triangle is zeroed
+// unless POTRF_BAND_ZERO asks for the band only.
 constexpr int POTRF_NO_SAVE = 1, POTRF_KEEP_UPPER = 2, POTRF_BAND_ZERO = 4, POTRF_ASYNC = 8,
-              POTRF_INFO_ZEROED = 16;
+              POTRF_INFO_ZEROED = 16, POTRF_INVERT = 32;
 int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, int max_tries,
           double jitter0, double* jitter_used, void* ws, size_t ws_bytes, hipStream_t s,
           int opts = 0);

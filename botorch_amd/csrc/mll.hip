@@ -365,11 +365,11 @@ static int enqueue_mll(const MllWs& w, int kind, int64_t n, int d, double s, dou
   int rc = covar_sym_lower(kind, xs, n, d, s, diag, L, np, np, st,
                            const_cast<int*>(potrf_status(fact, np)));
   if (rc) return rc;
+  // the factorisation's launches also build X = L^-1 (zero only in the 64 x 64
+  // blocks right of the diagonal blocks, all the 128-row tiles below read)
   rc = potrf(L, np, np, Linv, np, 0, 1e-8, nullptr, fact, fact_bytes, st,
              POTRF_NO_SAVE | POTRF_KEEP_UPPER | POTRF_BAND_ZERO | POTRF_ASYNC |
-                 POTRF_INFO_ZEROED);
-  if (rc) return rc;
-  rc = trtri_lower(L, np, Linv, np, np, fact, fact_bytes, st, TRTRI_UPPER_ZERO);
+                 POTRF_INFO_ZEROED | POTRF_INVERT);
   if (rc) return rc;
   hipLaunchKernelGGL(trmv_lower_kernel, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, Linv,
                      np, y, c, n, np, t, L, lgd);

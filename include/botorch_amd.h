@@ -159,6 +159,20 @@ int bta_potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv
               int32_t max_tries, double jitter0, double* jitter_used, void* ws,
               size_t ws_bytes, bta_stream_t stream);
 
+/* bta_potrf followed by the whole lower inverse: Linv = L^-1 (strict upper
+ * triangle zeroed), built by side tasks inside the factorisation's launches
+ * instead of a separate bta_trtri_lower pass.  Linv must be 16-byte aligned
+ * with an even ldinv.  The first call for a given n_pad on a device builds a
+ * task plan and keeps a device copy of it (a few hundred KiB at n_pad = 4096)
+ * until the process ends.  Replaces psd_safe_cholesky (as bta_potrf) plus the
+ * triangular inverse behind fast_pred_var's R = L^-T (gpytorch
+ * DefaultPredictionStrategy via models/gpytorch.py:585) and behind the exact
+ * marginal log likelihood's K^-1 (gpytorch ExactMarginalLogLikelihood, called
+ * from optim/closures/model_closures.py:133-142). */
+int bta_potrf_inverse(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv,
+                      int32_t max_tries, double jitter0, double* jitter_used, void* ws,
+                      size_t ws_bytes, bta_stream_t stream);
+
 /* Lower-triangular inverse: Linv = L^-1, where Linv already holds the inverted
  * 64x64 diagonal blocks (from bta_potrf).  Recursive doubling over block
  * pairs, two GEMMs per level.  Used for fast_pred_var's R = L^-T

@@ -168,6 +168,85 @@ def test_potrf_and_trtri(n):
     assert torch.triu(Linv, 1).abs().max().item() == 0.0
 
 
+@pytest.mark.parametrize("n,n_pad", [(64, 64), (100, 128), (200, 256), (300, 320),
+                                       (1000, 1088), (1500, 1536)])
+def test_potrf_inverse(n, n_pad):
+    """Cholesky with L^-1 built inside the factorisation's launches (odd block
+    counts included: 5 and 17 blocks of 64) against torch's FP64 Cholesky and
+    triangular solve, and against the separate inverse pass."""
+    _say(f"potrf_inverse n={n} n_pad={n_pad}")
+    L = _lib.lib()
+    A = _spd(n, n)
+    Ad = torch.eye(n_pad, dtype=torch.float64, device=DEV)
+    Ad[:n, :n] = A.to(DEV)
+    A2 = Ad.clone()
+    Linv = torch.full((n_pad, n_pad), 3.0, dtype=torch.float64, device=DEV)
+    ws = _lib.workspace(max(L.bta_potrf_workspace_size(n_pad), n_pad * n_pad * 8), DEV)
+    jit = ctypes.c_double(-1)
+    _lib.check(L.bta_potrf_inverse(_lib.ptr(Ad), n_pad, n_pad, _lib.ptr(Linv), n_pad, 6, 1e-8,
+                                   ctypes.byref(jit), _lib.ptr(ws), ws.numel(), _lib.stream()))
+    Lref = torch.linalg.cholesky(A)
+    torch.testing.assert_close(Ad[:n, :n].cpu(), Lref, rtol=1e-10, atol=1e-12)
+    assert jit.value == 0.0
+    assert torch.triu(Ad, 1).abs().max().item() == 0.0
+    Iref = torch.linalg.solve_triangular(Lref, torch.eye(n, dtype=torch.float64), upper=False)
+    torch.testing.assert_close(Linv[:n, :n].cpu(), Iref, rtol=1e-8, atol=1e-9 * Iref.abs().max())
+    assert torch.triu(Linv, 1).abs().max().item() == 0.0
+    # the padding block of the identity inverts to the identity
+    torch.testing.assert_close(Linv[n:, n:].cpu(), torch.eye(n_pad - n, dtype=torch.float64))
+    # the two-pass route (factorisation, then bta_trtri_lower) gives the same inverse
+    Linv2 = torch.zeros_like(Linv)
+    _lib.check(L.bta_potrf(_lib.ptr(A2), n_pad, n_pad, _lib.ptr(Linv2), n_pad, 6, 1e-8,
+                           ctypes.byref(jit), _lib.ptr(ws), ws.numel(), _lib.stream()))
+    _lib.check(L.bta_trtri_lower(_lib.ptr(A2), n_pad, _lib.ptr(Linv2), n_pad, n_pad,
+                                 _lib.ptr(ws), ws.numel(), _lib.stream()))
+    err = ((Linv - Linv2).abs().max() / Linv2.abs().max()).item()
+    assert err < 1e-11, err
+
+
+def test_potrf_inverse_large_and_jitter():
+    """n = 4096 (64 steps with side tasks in every one) twice on one workspace,
+    then a rank-deficient matrix that needs the jitter retry: the inverse must
+    be that of the jittered factor."""
+    _say("potrf_inverse n=4096 x2")
+    L = _lib.lib()
+    n = 4096
+    g = torch.Generator(device=DEV).manual_seed(11)
+    B = torch.randn(n, n, dtype=torch.float64, device=DEV, generator=g)
+    A0 = B @ B.T / n + 0.05 * torch.eye(n, dtype=torch.float64, device=DEV)
+    Lref = torch.linalg.cholesky(A0)
+    Iref = torch.linalg.solve_triangular(Lref, torch.eye(n, dtype=torch.float64, device=DEV),
+                                         upper=False)
+    ws = _lib.workspace(L.bta_potrf_workspace_size(n), DEV)
+    Linv = torch.empty_like(A0)
+    for _ in range(2):
+        A = A0.clone()
+        Linv.fill_(float("nan"))
+        jit = ctypes.c_double(-1)
+        _lib.check(L.bta_potrf_inverse(_lib.ptr(A), n, n, _lib.ptr(Linv), n, 6, 1e-8,
+                                       ctypes.byref(jit), _lib.ptr(ws), ws.numel(),
+                                       _lib.stream()))
+        err = ((Linv - Iref).abs().max() / Iref.abs().max()).item()
+        assert err < 1e-10, err
+    _say("potrf_inverse jitter retry n=100 (n_pad 192)")
+    gc = torch.Generator().manual_seed(5)
+    V = torch.randn(100, 30, generator=gc, dtype=torch.float64)
+    A = V @ V.t()
+    n_pad = 192
+    Ad = torch.eye(n_pad, dtype=torch.float64, device=DEV)
+    Ad[:100, :100] = A.to(DEV)
+    Linv = torch.full((n_pad, n_pad), float("nan"), dtype=torch.float64, device=DEV)
+    ws = _lib.workspace(L.bta_potrf_workspace_size(n_pad), DEV)
+    jit = ctypes.c_double(-1)
+    _lib.check(L.bta_potrf_inverse(_lib.ptr(Ad), n_pad, n_pad, _lib.ptr(Linv), n_pad, 6, 1e-8,
+                                   ctypes.byref(jit), _lib.ptr(ws), ws.numel(), _lib.stream()))
+    assert jit.value > 0.0
+    Lg = Ad.cpu()
+    Ig = torch.linalg.solve_triangular(Lg, torch.eye(n_pad, dtype=torch.float64), upper=False)
+    err = ((Linv.cpu() - Ig).abs().max() / Ig.abs().max()).item()
+    assert err < 1e-8, err
+
+
 def test_potrf_large_repeated():
     """n = 4096: the factorisation's step launches then hold more workgroups than
     CUs (strip and trailing-update workgroups share CUs); three runs must all
