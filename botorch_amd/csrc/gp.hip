@@ -223,10 +223,16 @@ int gp_cache_build(const bta_gp_model* m, const double* y, double* L, double* Li
   int rc = covar(m->kind, m->train_xs, m->n, m->train_xs, m->n, m->d, m->outputscale,
                  m->noise + m->chol_jitter, false, L, np, np, np, s, /*pad_identity=*/1);
   if (rc) return rc;
-  rc = potrf(L, np, np, Linv, np, max_tries, 1e-8, jitter_used, wf, fact_bytes, s);
+  // L^-1 built inside the factorisation's launches when Linv suits the
+  // direct-to-LDS loads (16-byte aligned), else by the separate inverse pass
+  const bool fused = ((uintptr_t)Linv & 15) == 0;
+  rc = potrf(L, np, np, Linv, np, max_tries, 1e-8, jitter_used, wf, fact_bytes, s,
+             fused ? POTRF_INVERT : 0);
   if (rc) return rc;
-  rc = trtri_lower(L, np, Linv, np, np, wf, fact_bytes, s);
-  if (rc) return rc;
+  if (!fused) {
+    rc = trtri_lower(L, np, Linv, np, np, wf, fact_bytes, s);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, y,
                      m->n, m->mean_constant, r, np);
   BTA_LAUNCH_CHECK("residual_kernel");

@@ -7,9 +7,12 @@
  *  - Matrices are row-major with an explicit leading dimension (elements).
  *  - "Scaled" inputs are (x - center) / lengthscale, coordinate-wise.
  *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = default
- *    stream) unless its comment says it synchronises; it allocates no device
- *    memory: scratch comes from the caller through (ws, ws_bytes) sized by the
- *    matching *_workspace_size function.
+ *    stream) unless its comment says it synchronises; scratch comes from the
+ *    caller through (ws, ws_bytes) sized by the matching *_workspace_size
+ *    function.  The one device allocation the library makes itself is the task
+ *    plan of bta_potrf_inverse (also used by bta_gp_cache_build and
+ *    bta_mll_value_grad): made on the first call for an n_pad on a device, held
+ *    until the process ends.
  *  - Return value: BTA_OK or a BTA_ERR_* code; bta_last_error() gives the text
  *    (thread-local).
  *
