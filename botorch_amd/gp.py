@@ -463,10 +463,17 @@ class SingleTaskGP(torch.nn.Module):
             mean, cov = _PosteriorFn.apply(Xf, self)
         else:
             mean, cov = self._posterior_moments(Xf.detach().contiguous())
-        if observation_noise:
+        if observation_noise is not False:
+            # the likelihood's noise (True) or the given per-point noise (a tensor
+            # batch x q x 1), in the model's standardized scale, added before the
+            # outcome untransform (models/gpytorch.py:514-538)
             _, y_std = self.y_stats
-            cov = cov + (y_std**2) * self.likelihood.noise.detach() * torch.eye(
-                mq, dtype=cov.dtype, device=dev)
+            if torch.is_tensor(observation_noise):
+                nz = observation_noise.to(cov).squeeze(-1).expand(*batch_shape, mq)
+                cov = cov + (y_std**2) * torch.diag_embed(nz.reshape(-1, mq))
+            else:
+                cov = cov + (y_std**2) * self.likelihood.noise.detach() * torch.eye(
+                    mq, dtype=cov.dtype, device=dev)
         mean = mean.reshape(*batch_shape, mq)
         cov = cov.reshape(*batch_shape, mq, mq)
         return GPyTorchPosterior(mean, cov)

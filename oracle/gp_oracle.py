@@ -218,10 +218,22 @@ class OracleGP:
         cov = Kxx - root.matmul(root.transpose(-1, -2))
         return mean, cov
 
-    def posterior(self, X: Tensor):
+    def posterior(self, X: Tensor, observation_noise=False):
         """Untransformed posterior (mean ... x m, covariance ... x m x m)
-        following Standardize.untransform_posterior (outcome.py:488-507)."""
+        following Standardize.untransform_posterior (outcome.py:488-507).
+        observation_noise (models/gpytorch.py:514-538, applied before the
+        untransform): True adds the likelihood's noise to the diagonal, a tensor
+        (... x m x 1) adds its own values."""
         mean, cov = self.posterior_std(X)
+        if observation_noise is not False:
+            m = X.shape[-2]
+            if torch.is_tensor(observation_noise):
+                nz = observation_noise.squeeze(-1).to(cov).expand(*cov.shape[:-1])
+            else:
+                nz = torch.full(cov.shape[:-1], float(self.noise), dtype=cov.dtype,
+                                device=cov.device)
+            cov = cov + torch.diag_embed(nz)
+            assert cov.shape[-1] == m
         return self.y_mean + self.y_std * mean, (self.y_std**2) * cov
 
 

@@ -59,6 +59,29 @@ def test_posterior_small(kind):
     assert _rel(post.variance.squeeze(-1).cpu(), torch.diagonal(c_ref, dim1=-2, dim2=-1)) < 1e-7
 
 
+def test_posterior_observation_noise():
+    """posterior(X, observation_noise=True / a per-point tensor) adds the noise on
+    the standardized scale before the untransform, as models/gpytorch.py:514-538."""
+    _say("posterior observation_noise gp_small")
+    w = workloads.gp_small()
+    gp, orc = _models(w, O.MATERN52, constant=0.3)
+    g = torch.Generator().manual_seed(13)
+    Xt = torch.rand(2, 5, 6, generator=g, dtype=torch.float64)
+    post = gp.posterior(Xt.to(DEV), observation_noise=True)
+    m_ref, c_ref = orc.posterior(Xt, observation_noise=True)
+    assert _rel(post.mean.squeeze(-1).cpu(), m_ref) < 1e-9
+    assert _rel(post.covariance_matrix.cpu(), c_ref) < 1e-7
+    noise = 0.01 + torch.rand(2, 5, 1, generator=g, dtype=torch.float64)
+    post = gp.posterior(Xt.to(DEV), observation_noise=noise.to(DEV))
+    m_ref, c_ref = orc.posterior(Xt, observation_noise=noise)
+    assert _rel(post.covariance_matrix.cpu(), c_ref) < 1e-7
+    # the noise-free covariance plus the noise on the diagonal, nothing else
+    _, c0 = orc.posterior(Xt)
+    off = c_ref - c0
+    assert _rel(torch.diagonal(off, dim1=-2, dim2=-1), noise.squeeze(-1) * orc.y_std**2) < 1e-12
+    assert (off - torch.diag_embed(torch.diagonal(off, dim1=-2, dim2=-1))).abs().max() == 0
+
+
 def test_posterior_larger_n():
     _say("posterior n=300 d=5 batch 2 x 70")
     g = torch.Generator().manual_seed(12)
