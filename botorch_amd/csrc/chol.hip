@@ -889,6 +889,11 @@ int potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv, in
     set_error("potrf: the inverse needs Linv 16-byte aligned with an even leading dimension");
     return BTA_ERR_INVALID;
   }
+  if (invert && (const char*)Linv < (const char*)(A + (n_pad - 1) * lda + n_pad) &&
+      (const char*)A < (const char*)(Linv + (n_pad - 1) * ldinv + n_pad)) {
+    set_error("potrf: Linv overlaps A");
+    return BTA_ERR_INVALID;
+  }
   double* saved = (double*)ws;
   double* wblk = saved + n_pad * n_pad;
   int* info = (int*)(wblk + LEAF * LEAF);

@@ -1,5 +1,6 @@
 """The C-ABI library loads without a GPU and exports exactly the entry points
 include/botorch_amd.h declares, with the signatures the ctypes binding uses."""
+import ctypes
 import os
 import re
 
@@ -34,7 +35,6 @@ def test_library_metadata_without_gpu():
 
 
 def test_workspace_sizes_are_consistent():
-    import ctypes
     lib = _lib.load()
     assert lib.bta_potrf_workspace_size(256) >= 256 * 256 * 8
     assert lib.bta_mll_workspace_size(4096, 16) >= 2 * 4096 * 4096 * 8
@@ -53,3 +53,16 @@ def test_invalid_arguments_return_error_not_crash():
                       0, 0, None)
     assert rc == _lib.BTA_ERR_INVALID
     assert b"bta_gemm" in lib.bta_last_error()
+    # the fused Cholesky + inverse refuses its operands before touching the device
+    # (pointers are never dereferenced on these paths)
+    jit = ctypes.c_double(0)
+    ws_n = lib.bta_potrf_workspace_size(128)
+    A, ws = ctypes.c_void_p(1 << 20), ctypes.c_void_p(1 << 30)
+    rc = lib.bta_potrf_inverse(A, 128, 128, None, 128, 6, 1e-8, ctypes.byref(jit), ws, ws_n, None)
+    assert rc == _lib.BTA_ERR_INVALID and b"null" in lib.bta_last_error()
+    rc = lib.bta_potrf_inverse(A, 128, 128, ctypes.c_void_p((1 << 20) + 64 * 8), 128, 6, 1e-8,
+                               ctypes.byref(jit), ws, ws_n, None)
+    assert rc == _lib.BTA_ERR_INVALID and b"overlaps" in lib.bta_last_error()
+    rc = lib.bta_potrf_inverse(A, 128, 128, ctypes.c_void_p((1 << 24) + 8), 128, 6, 1e-8,
+                               ctypes.byref(jit), ws, ws_n, None)
+    assert rc == _lib.BTA_ERR_INVALID and b"aligned" in lib.bta_last_error()
