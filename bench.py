@@ -41,6 +41,12 @@ def _timed(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def _timed_median(fn, steps, warmup, reps=3):
+    """Median over reps back-to-back timings of steps calls (after warmup)."""
+    ts = [_timed(fn, steps, warmup if r == 0 else 0) for r in range(reps)]
+    return sorted(ts)[len(ts) // 2]
+
+
 def build_headline(precision: str, n=2048, d=12, b=512, q=4, S=512):
     """The product acquisition of the headline workload (inputs on the GPU)."""
     import botorch_amd as B
@@ -442,7 +448,7 @@ def main():
                                             0.05**2, 0.0, 6, outb, _lib.ptr(ws), ws.numel(),
                                             _lib.stream()))
 
-        t_mll = _timed(mll_step, 10, 2)
+        t_mll = _timed_median(mll_step, 10, 5)
         g_hip = torch.tensor(list(outb), dtype=torch.float64)
         log(f"MLL hip: {t_mll * 1e3:.2f} ms")
         lsr = torch.full((dm,), 0.4, dtype=torch.float64, device="cuda", requires_grad=True)
@@ -462,7 +468,7 @@ def main():
         g_ref_m = torch.cat([torch.stack([ll.detach(), osr.grad, nr.grad, cr.grad]),
                              lsr.grad]).cpu()
         if not args.no_baselines:
-            t_mll_torch = _timed(torch_mll, 3, 1)
+            t_mll_torch = _timed_median(torch_mll, 3, 1)
             log(f"MLL torch fp64: {t_mll_torch * 1e3:.2f} ms")
         mll = {"workload": "exact MLL value+grad, n=4096, d=16, Matern-5/2 ARD",
                "mll_grad_per_s": 1.0 / t_mll, "ms": t_mll * 1e3,

@@ -87,12 +87,12 @@ __global__ __launch_bounds__(256) void covar_kernel(const double* __restrict__ X
 // Symmetric case for the factorisation: K = s k(Xs, Xs) + diag_add I on and
 // below the diagonal only (the Cholesky reads nothing above it), identity on
 // the padding, kernel kind fixed at compile time.  Blocks wholly above the
-// diagonal exit at once.  The block's ROWS row points and its 256 column
-// points are staged in LDS by coalesced loads (a column point's coordinates
-// are contiguous, so lane-per-column global loads would touch a cache line per
-// lane); the odd pitch MD + 1 keeps the per-lane column reads conflict-free.
-// When zero2 is set, the first block also zeroes those two ints (the status
-// pair of the factorisation that follows).
+// diagonal exit at once.  The block's ROWS row points are staged in LDS (read
+// as wave-wide broadcasts); each lane loads its own column point's coordinates
+// straight into registers (staging the 256 column points in LDS as well
+// measured 64.5 against 59.3 us at n = 4096, d = 16).  When zero2 is set, the
+// first block also zeroes those two ints (the status pair of the
+// factorisation that follows).
 template <int KIND, int MD, int ROWS>
 __global__ __launch_bounds__(256) void covar_sym_lower_kernel(const double* __restrict__ Xs,
                                                               int64_t n, int d, double s,
@@ -101,7 +101,6 @@ __global__ __launch_bounds__(256) void covar_sym_lower_kernel(const double* __re
                                                               int64_t ldk, int64_t n_pad,
                                                               int* zero2) {
   __shared__ double xr[ROWS * MD];
-  __shared__ double xc[256 * (MD + 1)];
   if (zero2 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) zero2[threadIdx.x] = 0;
   const int64_t r0 = (int64_t)blockIdx.y * ROWS;
   const int64_t c0 = (int64_t)blockIdx.x * 256;
@@ -112,17 +111,12 @@ __global__ __launch_bounds__(256) void covar_sym_lower_kernel(const double* __re
     const int j = i % MD;
     xr[i] = (r < n && j < d) ? Xs[r * d + j] : 0.0;
   }
-  for (int i = threadIdx.x; i < 256 * MD; i += 256) {
-    const int64_t cc = c0 + i / MD;
-    const int j = i % MD;
-    xc[(i / MD) * (MD + 1) + j] = (cc < n && j < d) ? Xs[cc * d + j] : 0.0;
-  }
   __syncthreads();
   if (c >= n_pad) return;
   double t[MD];
   const bool cvalid = c < n;
 #pragma unroll
-  for (int j = 0; j < MD; ++j) t[j] = xc[threadIdx.x * (MD + 1) + j];
+  for (int j = 0; j < MD; ++j) t[j] = (cvalid && j < d) ? Xs[c * d + j] : 0.0;
 #pragma unroll 2
   for (int rr = 0; rr < ROWS; ++rr) {
     const int64_t r = r0 + rr;
