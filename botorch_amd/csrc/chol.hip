@@ -15,6 +15,13 @@
 // diagonal factorisation.  trinv_diag_kernel then inverts every 64 x 64
 // diagonal block into Linv.
 //
+// bta_potrf_inverse (POTRF_INVERT) builds all of Linv = L^-1 in the same
+// launches: the non-strip workgroups first run the step's side tasks (partial
+// products of the row-block recurrence of L^-1, accumulated in place in Linv,
+// and the finishing products by the inverted diagonal blocks), placed by the
+// host plan of inv_plan.h so that every operand is final before its launch,
+// then the trailing tiles; two more launches carry the last rows' tasks.
+//
 // bta_trtri_lower inverts L bottom-up by recursive doubling: with the 64x64
 // diagonal inverses in place, each level joins pairs of inverted diagonal
 // blocks [A 0; C B] -> [A^-1 0; -B^-1 C A^-1 B^-1] with two MFMA GEMMs batched
