@@ -8,9 +8,10 @@
 // lower 128 x 64 tile of it on the FP64 MFMA main loop (d_mainloop, both
 // operands read K-major straight from X), the K range of a tile,
 // [max(128 tm, 64 tn), n_pad), cut into chunks of KC so that no task grows
-// without bound at large n; the workers pair long and short tasks on each
-// CU.  Each task's epilogue rebuilds k and dk/dtheta for the tile's point
-// pairs from the scaled inputs staged in LDS and folds -P_ij dK_ij/dtheta
+// without bound at large n; a host-built schedule deals the tasks, longest
+// first, to the least loaded CU (trace_schedule).  Each task's epilogue
+// rebuilds k and dk/dtheta for the tile's point pairs from the scaled inputs
+// staged in LDS and folds -P_ij dK_ij/dtheta
 // (plus a_i a_j dK_ij/dtheta for the tile's first chunk; the sum is linear in
 // K^-1) into per-worker sums, so the n^2 x (d + 2) derivative tensor is never
 // formed either.  a comes from two triangular matrix-vector products with X.
@@ -25,6 +26,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <array>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace bta {
@@ -38,6 +42,7 @@ struct TraceArgs {
   const double* xs;     // n x d scaled inputs
   const double* alpha;  // n_pad
   double* partial;      // gridDim.x x (MD + 2)
+  const int* sched;     // worker w's tasks: sched[W + 1 + i], i in [sched[w], sched[w + 1])
   int n_pad, n, d, MT, CB;
   double s;
 };
@@ -54,7 +59,7 @@ __host__ __device__ inline int trace_tasks(int MT, int CB) {
   }
   return total;
 }
-__device__ __forceinline__ bool trace_task(int t, int MT, int CB, int& tm, int& tn, int& c) {
+__host__ __device__ inline bool trace_task(int t, int MT, int CB, int& tm, int& tn, int& c) {
   for (c = 0;; ++c) {
     const int T = MT - c * CB;
     if (T <= 0) return false;
@@ -85,8 +90,9 @@ __global__ __launch_bounds__(256, (MD == 16 ? 2 : 1)) void mll_trace_kernel(Trac
 #pragma unroll
   for (int j = 0; j < MD; ++j) acc_ls[j] = 0.0;
 
-  for (int jj = 0; jj * W < total; ++jj) {
-    const int t = h2_tile_of(jj, w, W);
+  const int* list = a.sched + W + 1;
+  for (int i = a.sched[w]; i < a.sched[w + 1]; ++i) {
+    const int t = list[i];
     int tm, tn, c;
     if (t >= total || !trace_task(t, a.MT, a.CB, tm, tn, c)) continue;
     const int kb = max(tm * TR_B, tn * TN) + c * TR_KC;
@@ -312,6 +318,78 @@ __global__ __launch_bounds__(256) void mll_finish_kernel(const double* t, const 
 }
 
 
+// Static schedule of the trace tasks over W workers.  Tasks, longest first
+// (cost: the K length plus TRACE_EPI for the epilogue's kernel values), go one
+// by one to the least loaded CU and there to its less loaded worker; with
+// W = 2 x CUs the dispatcher places workers w and w + W/2 on one CU, which
+// then runs their sum of work, otherwise every worker has a CU of its own.
+// (The h2 heavy/light dealing it replaces left the busiest CU 16 % above the
+// mean at n = 4096; this schedule, 3 %.)  Built on the host once per (device,
+// n_pad, W) and kept on the device until the process ends.
+constexpr int TRACE_EPI = 160;
+
+static std::vector<int> trace_schedule_host(int MT, int CB, int n_pad, int W, int ncu) {
+  const int total = trace_tasks(MT, CB);
+  std::vector<std::pair<int, int>> cost(total);  // (cost, task)
+  for (int t = 0; t < total; ++t) {
+    int tm, tn, c;
+    trace_task(t, MT, CB, tm, tn, c);
+    const int kb = std::max(tm * TR_B, tn * 64) + c * TR_KC;
+    const int ke = std::min(n_pad, kb + TR_KC);
+    cost[t] = {std::max(ke - kb, 0) + TRACE_EPI, t};
+  }
+  std::stable_sort(cost.begin(), cost.end(),
+                   [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+                     return x.first > y.first;
+                   });
+  const bool paired = W == 2 * ncu;
+  const int bins = paired ? W / 2 : W;
+  std::vector<int64_t> bl(bins, 0), wl(W, 0);
+  std::vector<std::vector<int>> lists(W);
+  for (const auto& ct : cost) {
+    int b = 0;
+    for (int i = 1; i < bins; ++i)
+      if (bl[i] < bl[b]) b = i;
+    int w = b;
+    if (paired && wl[b + bins] < wl[b]) w = b + bins;
+    bl[b] += ct.first;
+    wl[w] += ct.first;
+    lists[w].push_back(ct.second);
+  }
+  std::vector<int> out(W + 1 + total);
+  int k = W + 1;
+  for (int w = 0; w < W; ++w) {
+    out[w] = k - (W + 1);
+    for (int t : lists[w]) out[k++] = t;
+  }
+  out[W] = total;
+  return out;
+}
+
+static int trace_schedule(int MT, int CB, int n_pad, int W, const int** out) {
+  static std::mutex& mu = *new std::mutex;  // never destroyed
+  static std::map<std::array<int, 3>, const int*>& cache =
+      *new std::map<std::array<int, 3>, const int*>;
+  int dev = 0;
+  BTA_HIP(hipGetDevice(&dev));
+  const std::array<int, 3> key = {dev, n_pad, W};
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    const std::vector<int> h = trace_schedule_host(MT, CB, n_pad, W, num_cus());
+    void* p = nullptr;
+    BTA_HIP(hipMalloc(&p, h.size() * sizeof(int)));
+    const hipError_t e = hipMemcpy(p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return hip_fail(e, "hipMemcpy(trace schedule)");
+    }
+    it = cache.emplace(key, (const int*)p).first;
+  }
+  *out = it->second;
+  return BTA_OK;
+}
+
 struct MllWs {
   int64_t n_pad;
   int W, MD;
@@ -397,6 +475,8 @@ static int enqueue_mll(const MllWs& w, int kind, int64_t n, int d, double s, dou
   ta.s = s;
   const int tasks = trace_tasks(ta.MT, ta.CB);
   const int W = std::min(w.W, tasks);
+  rc = trace_schedule(ta.MT, ta.CB, (int)np, W, &ta.sched);
+  if (rc) return rc;
   if (d <= 16)
     launch_trace<16>(kind, ta, W, st);
   else

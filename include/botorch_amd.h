@@ -9,10 +9,11 @@
  *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = default
  *    stream) unless its comment says it synchronises; scratch comes from the
  *    caller through (ws, ws_bytes) sized by the matching *_workspace_size
- *    function.  The one device allocation the library makes itself is the task
- *    plan of bta_potrf_inverse (also used by bta_gp_cache_build and
- *    bta_mll_value_grad): made on the first call for an n_pad on a device, held
- *    until the process ends.
+ *    function.  The device allocations the library makes itself are two small
+ *    tables, each made on the first call for an n_pad on a device and held
+ *    until the process ends: the task plan of bta_potrf_inverse (also used by
+ *    bta_gp_cache_build and bta_mll_value_grad) and the trace kernel's worker
+ *    schedule of bta_mll_value_grad.
  *  - Return value: BTA_OK or a BTA_ERR_* code; bta_last_error() gives the text
  *    (thread-local).
  *
