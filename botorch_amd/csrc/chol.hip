@@ -678,13 +678,18 @@ __device__ __forceinline__ void potrf_work(const StepArgs& a, unsigned char* sme
     ntile = MT * (MT + 1);
   }
   const int total = a.nside + ntile;
-  const int W = gridDim.x - a.nstrip;
-  // task t = w, w + W, ...: the partial tasks (the plan's first npart), the
-  // finish tasks, then the tiles, each kind in a loop of its own
-  int t = blockIdx.x - a.nstrip;
-  for (; t < a.npart; t += W) inv_partial(a, a.side[t], smem);
-  for (; t < a.nside; t += W) inv_finish(a, a.side[t], smem);
-  for (; t < total; t += W) potrf_bulk_tile(a, t - a.nside, rem, smem);
+  const int W = gridDim.x - a.nstrip, w = blockIdx.x - a.nstrip;
+  // Worker w takes task k W + w in even rounds k and k W + (W - 1 - w) in odd
+  // ones, so the workers that drew the last (lightest) tasks of a round draw
+  // the first of the next.  Tasks run in order: the partial tasks (the plan's
+  // first npart, longest first), the finish tasks, then the tiles, each kind in
+  // a loop of its own (a worker's task indices increase).
+  int k = 0;
+  auto task = [&](int kk) { return kk * W + ((kk & 1) ? W - 1 - w : w); };
+  int t = task(k);
+  for (; t < a.npart; t = task(++k)) inv_partial(a, a.side[t], smem);
+  for (; t < a.nside; t = task(++k)) inv_finish(a, a.side[t], smem);
+  for (; t < total; t = task(++k)) potrf_bulk_tile(a, t - a.nside, rem, smem);
 }
 
 __global__ __launch_bounds__(256, 2) void potrf_step_kernel(StepArgs a) {
