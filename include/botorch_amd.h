@@ -166,9 +166,11 @@ int bta_potrf(double* A, int64_t n_pad, int64_t lda, double* Linv, int64_t ldinv
 /* bta_potrf followed by the whole lower inverse: Linv = L^-1 (strict upper
  * triangle zeroed), built by side tasks inside the factorisation's launches
  * instead of a separate bta_trtri_lower pass.  Linv must be 16-byte aligned
- * with an even ldinv.  The first call for a given n_pad on a device builds a
- * task plan and keeps a device copy of it (a few hundred KiB at n_pad = 4096)
- * until the process ends.  Replaces psd_safe_cholesky (as bta_potrf) plus the
+ * with an even ldinv and must not overlap A (BTA_ERR_INVALID otherwise).  The
+ * jitter schedule, errors and workspace are bta_potrf's; like it, the call
+ * SYNCHRONISES on `stream` (reads info).  The first call for a given n_pad on a
+ * device builds a task plan and keeps a device copy of it (201 KB at
+ * n_pad = 4096) until the process ends.  Replaces psd_safe_cholesky (as bta_potrf) plus the
  * triangular inverse behind fast_pred_var's R = L^-T (gpytorch
  * DefaultPredictionStrategy via models/gpytorch.py:585) and behind the exact
  * marginal log likelihood's K^-1 (gpytorch ExactMarginalLogLikelihood, called
