@@ -165,8 +165,14 @@ __device__ __forceinline__ double wave_sum_spread(const double (&v)[NV], int* e_
     const bool up = (lane & off) != 0;
 #pragma unroll
     for (int i = 0; i < h; ++i) {
-      const double keep = up ? a[h + i] : a[i];
-      const double send = up ? a[i] : a[h + i];
+      // The empty asm (no instruction) makes lo and hi opaque values: without
+      // it the compiler turns the lane-dependent choice below into a select of
+      // the element index and extracts a[index] through compare/select chains
+      // over the whole array (about 7x the instructions at NV = 10).
+      double lo = a[i], hi = a[h + i];
+      __asm__("" : "+v"(lo), "+v"(hi));
+      const double keep = up ? hi : lo;
+      const double send = up ? lo : hi;
       a[i] = keep + __shfl_xor(send, off, 64);
     }
   }
