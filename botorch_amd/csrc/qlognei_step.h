@@ -65,30 +65,46 @@ struct McLayout {
   }
 };
 
+// a / b in TM: FP32 takes the hardware reciprocal (1 ulp) and a product, FP64
+// divides exactly
+template <typename TM>
+__device__ __forceinline__ TM div_tm(TM a, TM b) {
+  if constexpr (sizeof(TM) == 4) return a * __builtin_amdgcn_rcpf(b);
+  else return a / b;
+}
+
 // log(tau * (softplus(x/tau) + 0.1 / (1 + (x/tau)^2))) and its derivative
 // (utils/safe_math.py:298-325), evaluated in TM (FP64, or FP32 for the mixed
-// modes' sample arithmetic) from the FP64 difference x.
+// modes' sample arithmetic) from the FP64 difference x; rtau = 1 / tau.  In
+// FP32, log1p(e) follows Goldberg's rule log(u) e / (u - 1), u = 1 + e (a few
+// ulp; "What every computer scientist should know about floating-point
+// arithmetic", 1991, thm. 4), and the divisions use div_tm.
 template <typename TM>
-__device__ __forceinline__ void log_fatplus_grad(double x, double tau, double* v, double* dv) {
-  const TM z = (TM)(x / tau);
+__device__ __forceinline__ void log_fatplus_grad(double x, double tau, double rtau, double* v,
+                                                 double* dv) {
+  constexpr bool f32 = sizeof(TM) == 4;
+  const TM z = f32 ? (TM)(x * rtau) : (TM)(x / tau);
   TM sp, sig;
   if (z > (TM)20) {  // torch softplus threshold
     sp = z;
     sig = (TM)1;
   } else {
     const TM e = exp(z);
-    sp = log1p(e);
-    sig = e / ((TM)1 + e);
+    const TM u = (TM)1 + e;
+    if constexpr (f32) sp = u == (TM)1 ? e : log(u) * div_tm(e, u - (TM)1);
+    else sp = log1p(e);
+    sig = div_tm(e, u);
   }
   const TM z2 = (TM)1 + z * z;
-  const TM cz = (TM)1 / z2;
+  const TM cz = div_tm((TM)1, z2);
   const TM f = sp + (TM)0.1 * cz;
   const TM tt = (TM)tau;
   *v = (double)log(tt * f);
   // dv = (sig - 0.2 z cz^2) / (f tau), with cz / f = 1 / (sp (1 + z^2) + 0.1) so
   // that no intermediate underflows for large |z|
-  const TM cf = sp > (TM)0 ? (TM)1 / (sp * z2 + (TM)0.1) : (TM)10;
-  *dv = (double)((sig / f - (TM)0.2 * z * cz * cf) / tt);
+  const TM cf = sp > (TM)0 ? div_tm((TM)1, sp * z2 + (TM)0.1) : (TM)10;
+  const TM g = div_tm(sig, f) - (TM)0.2 * z * cz * cf;
+  *dv = (double)(f32 ? g * (TM)rtau : g / tt);
 }
 
 // block-wide reduction of NV values (NV <= 64); every thread gets the result
@@ -192,6 +208,18 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   const TV* vt = (const TV*)a.vt;
   const int64_t r0 = (int64_t)b * Q;
 
+  // the restart's V rows for V_b V_b^T (16-byte loads): the first two column
+  // passes are issued here, ahead of the other loads of the stage
+  constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
+  typedef TV vec_t __attribute__((ext_vector_type(VW)));
+  vec_t vpre[2][Q];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = (tid + 256 * h) * VW;
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      vpre[h][i] = k < a.n_pad ? *(const vec_t*)(vt + (r0 + i) * a.ldv + k) : vec_t{};
+  }
   for (int i = tid; i < Q * d; i += 256) xq[(i / d) * MC_MAXD + i % d] = a.xs[(r0 + i / d) * d + i % d];
   if (tid == 0) scal[1] = 0.0;
 
@@ -219,16 +247,10 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
       const double tot = wave_sum_spread<4>(s, &e);
       if (e >= 0 && t0 + e < Q * ne) pv[t0 + e] = tot;
     }
-    constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
-    typedef TV vec_t __attribute__((ext_vector_type(VW)));
     double acc[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) acc[i] = 0.0;
-#pragma unroll 2
-    for (int k = tid * VW; k < a.n_pad; k += 256 * VW) {
-      vec_t v[Q];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) v[i] = *(const vec_t*)(vt + (r0 + i) * a.ldv + k);
+    auto vv_pass = [&](const vec_t(&v)[Q]) {
 #pragma unroll
       for (int e = 0; e < VW; ++e) {
         int idx = 0;
@@ -240,6 +262,15 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
             ++idx;
           }
       }
+    };
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if ((tid + 256 * h) * VW < a.n_pad) vv_pass(vpre[h]);
+    for (int k = (tid + 512) * VW; k < a.n_pad; k += 256 * VW) {
+      vec_t v[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) v[i] = *(const vec_t*)(vt + (r0 + i) * a.ldv + k);
+      vv_pass(v);
     }
     double tot[NQ];
     block_sum<NQ>(acc, red, tot);
@@ -361,43 +392,49 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   // The per-sample transcendental arithmetic runs in TM: FP64 in the FP64
   // mode, FP32 in the mixed modes (samples and best_f stay FP64).
   typedef TV TM;
-  const double tr = a.tau_relu, tm = a.tau_max;
+  const double tr = a.tau_relu, tm = a.tau_max, rtr = 1.0 / tr, rtm = 1.0 / tm;
   bool bad = false;
 #pragma unroll 2
   for (int s = tid; s < S; s += 256) {
+    const double bf = a.best_f[s];
     double y[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) y[i] = mu[i];
-    for (int k = 0; k < m; ++k) {
-      const double z = a.Zt[(int64_t)k * S + s];
+    // base samples eight at a time, their loads issued before the products
+    for (int k0 = 0; k0 < m; k0 += 8) {
+      double z[8];
 #pragma unroll
-      for (int i = 0; i < Q; ++i) y[i] = fma(lnew[i * m + k], z, y[i]);
+      for (int u = 0; u < 8; ++u) z[u] = k0 + u < m ? a.Zt[(int64_t)(k0 + u) * S + s] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < m)
+#pragma unroll
+          for (int i = 0; i < Q; ++i) y[i] = fma(lnew[i * m + k0 + u], z[u], y[i]);
     }
-    const double bf = a.best_f[s];
     double v[Q], dv[Q];
     double M = -INFINITY;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       bad |= !isfinite(y[i]);
-      log_fatplus_grad<TM>(y[i] - bf, tr, &v[i], &dv[i]);
+      log_fatplus_grad<TM>(y[i] - bf, tr, rtr, &v[i], &dv[i]);
       M = fmax(M, v[i]);
     }
     TM P = 0, sdp = 0, dp[Q];
     int nties = 0;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const TM z = (TM)((M - v[i]) / tm);
+      const TM z = sizeof(TM) == 4 ? (TM)((M - v[i]) * rtm) : (TM)((M - v[i]) / tm);
       const TM den = (TM)2 + (TM)2 * z + z * z;
-      P += (TM)2 / den;
-      dp[i] = (TM)-2 * ((TM)2 + (TM)2 * z) / (den * den);
+      P += div_tm((TM)2, den);
+      dp[i] = div_tm((TM)-2 * ((TM)2 + (TM)2 * z), den * den);
       sdp += dp[i];
       nties += (v[i] == M);
     }
     uu[s] = M + tm * (double)log(P);
-    const TM cm = ((TM)1 + sdp / P) / (TM)nties;
+    const TM cm = div_tm((TM)1 + div_tm(sdp, P), (TM)nties);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const TM du = (v[i] == M ? cm : (TM)0) - dp[i] / P;
+      const TM du = (v[i] == M ? cm : (TM)0) - div_tm(dp[i], P);
       G[s * Q + i] = (double)du * dv[i];
     }
   }
@@ -409,7 +446,11 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   for (int s = tid; s < S; s += 256) lmax = fmax(lmax, uu[s]);
   const double Mu = block_max1(lmax, red);
   double lsum[1] = {0.0};
-  for (int s = tid; s < S; s += 256) lsum[0] += exp(uu[s] - Mu);
+  for (int s = tid; s < S; s += 256) {  // uu keeps exp(u - max) for the adjoint
+    const double e = exp(uu[s] - Mu);
+    uu[s] = e;
+    lsum[0] += e;
+  }
   double tsum[1];
   block_sum<1>(lsum, red, tsum);
   if (tid == 0) {
@@ -428,7 +469,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
 #pragma unroll
     for (int e = 0; e < NV; ++e) acc[e] = 0.0;
     for (int s = tid; s < S; s += 256) {
-      const double w = exp(uu[s] - Mu) / tsum[0];
+      const double w = uu[s] / tsum[0];
       double gw[Q];
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
@@ -461,7 +502,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
     __syncthreads();
   } else {
   for (int s = tid; s < S; s += 256) {
-    const double w = exp(uu[s] - Mu) / tsum[0];
+    const double w = uu[s] / tsum[0];
 #pragma unroll
     for (int i = 0; i < Q; ++i) G[s * Q + i] *= w;
   }
