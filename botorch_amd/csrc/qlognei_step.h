@@ -394,23 +394,8 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   typedef TV TM;
   const double tr = a.tau_relu, tm = a.tau_max, rtr = 1.0 / tr, rtm = 1.0 / tm;
   bool bad = false;
-#pragma unroll 2
-  for (int s = tid; s < S; s += 256) {
-    const double bf = a.best_f[s];
-    double y[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) y[i] = mu[i];
-    // base samples eight at a time, their loads issued before the products
-    for (int k0 = 0; k0 < m; k0 += 8) {
-      double z[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) z[u] = k0 + u < m ? a.Zt[(int64_t)(k0 + u) * S + s] : 0.0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (k0 + u < m)
-#pragma unroll
-          for (int i = 0; i < Q; ++i) y[i] = fma(lnew[i * m + k0 + u], z[u], y[i]);
-    }
+  // y -> log improvement -> fat-max -> u[s], and the per-sample adjoint G[s]
+  auto sample_tail = [&](const int s, const double bf, const double(&y)[Q]) {
     double v[Q], dv[Q];
     double M = -INFINITY;
 #pragma unroll
@@ -437,6 +422,39 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
       const TM du = (v[i] == M ? cm : (TM)0) - div_tm(dp[i], P);
       G[s * Q + i] = (double)du * dv[i];
     }
+  };
+  // Each thread carries the samples s0 and s0 + 256 through the products: the
+  // rows of [Lbl | Lbr] (LDS) and the base samples are read eight columns at a
+  // time at clamped indices, so that every read issues before the products;
+  // the products skip the columns past m.
+  for (int s0 = tid; s0 < S; s0 += 512) {
+    const bool two = s0 + 256 < S;
+    const int s1 = two ? s0 + 256 : s0;
+    const double bf0 = a.best_f[s0], bf1 = a.best_f[s1];
+    double y0[Q], y1[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) y0[i] = y1[i] = mu[i];
+    for (int k0 = 0; k0 < m; k0 += 8) {
+      double lc[Q][8], z0[8], z1[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u < m ? k0 + u : m - 1;
+        z0[u] = a.Zt[(int64_t)k * S + s0];
+        z1[u] = a.Zt[(int64_t)k * S + s1];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) lc[i][u] = lnew[i * m + k];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < m)
+#pragma unroll
+          for (int i = 0; i < Q; ++i) {
+            y0[i] = fma(lc[i][u], z0[u], y0[i]);
+            y1[i] = fma(lc[i][u], z1[u], y1[i]);
+          }
+    }
+    sample_tail(s0, bf0, y0);
+    if (two) sample_tail(s0 + 256, bf1, y1);
   }
   if (bad) scal[1] = 1.0;  // every writer stores the same value
   __syncthreads();
@@ -469,6 +487,10 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
 #pragma unroll
     for (int e = 0; e < NV; ++e) acc[e] = 0.0;
     for (int s = tid; s < S; s += 256) {
+      double z[MREG];  // loaded ahead of the products (uniform branches)
+#pragma unroll
+      for (int k = 0; k < MREG; ++k)
+        if (k < m) z[k] = a.Zt[(int64_t)k * S + s];
       const double w = uu[s] / tsum[0];
       double gw[Q];
 #pragma unroll
@@ -479,10 +501,9 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
 #pragma unroll
       for (int k = 0; k < MREG; ++k) {
         if (k < m) {
-          const double z = a.Zt[(int64_t)k * S + s];
 #pragma unroll
           for (int i = 0; i < Q; ++i)
-            acc[i * (MREG + 1) + k] = fma(gw[i], z, acc[i * (MREG + 1) + k]);
+            acc[i * (MREG + 1) + k] = fma(gw[i], z[k], acc[i * (MREG + 1) + k]);
         }
       }
     }
