@@ -208,8 +208,12 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   const TV* vt = (const TV*)a.vt;
   const int64_t r0 = (int64_t)b * Q;
 
-  // the restart's V rows for V_b V_b^T (16-byte loads): the first two column
-  // passes are issued here, ahead of the other loads of the stage
+  // the query coordinates (one per thread), then the restart's V rows for
+  // V_b V_b^T (16-byte loads): their first two column passes are issued here,
+  // ahead of the other loads of the stage
+  static_assert(Q * MC_MAXD <= 256, "one query coordinate per thread");
+  const bool has_x = tid < Q * d;
+  const double xr = has_x ? a.xs[r0 * d + tid] : 0.0;
   constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
   typedef TV vec_t __attribute__((ext_vector_type(VW)));
   vec_t vpre[2][Q];
@@ -220,8 +224,22 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
     for (int i = 0; i < Q; ++i)
       vpre[h][i] = k < a.n_pad ? *(const vec_t*)(vt + (r0 + i) * a.ldv + k) : vec_t{};
   }
-  for (int i = tid; i < Q * d; i += 256) xq[(i / d) * MC_MAXD + i % d] = a.xs[(r0 + i / d) * d + i % d];
+  if (has_x) xq[(tid / d) * MC_MAXD + tid % d] = xr;
   if (tid == 0) scal[1] = 0.0;
+  __syncthreads();
+  // Kernel values K(x_i, x_j) into sqq and K(x_i, X_b) into lnew[:, :n_b]; they
+  // need only the coordinates, so they run (from the last wave down) while the
+  // V rows and the P partials are in flight.
+  for (int t = 255 - tid; t < Q * Q + Q * n_b; t += 256) {
+    if (t < Q * Q) {
+      const int i = t / Q, j = t % Q;
+      sqq[t] = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d), a.outputscale);
+    } else {
+      const int e = t - Q * Q, i = e / n_b, kb = e % n_b;
+      lnew[i * m + kb] =
+          kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d), a.outputscale);
+    }
+  }
 
   // ---- P rows (fixed-order sums of their column-tile partials, one value per
   // wave task, partials across the lanes) and V_b V_b^T over the n_pad
@@ -280,18 +298,14 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   if (tid < Q * Q) {
     const int i = tid / Q, j = tid % Q;
     const int ii = i > j ? i : j, jj = i > j ? j : i;
-    const double kqq = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d),
-                                  a.outputscale);
-    sqq[i * Q + j] = ys2 * (kqq - vv[ii * (ii + 1) / 2 + jj]);
+    sqq[i * Q + j] = ys2 * (sqq[i * Q + j] - vv[ii * (ii + 1) / 2 + jj]);
   }
   if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + pv[tid * (n_b + 1)]);
   // ---- Sigma_qb into lnew[:, :n_b] (and a copy in dsqb for the joint mode)
   const bool joint = a.Lb_levels != nullptr;
   for (int e = tid; e < Q * n_b; e += 256) {
     const int i = e / n_b, kb = e % n_b;
-    const double kqb = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d),
-                                  a.outputscale);
-    const double sqb = ys2 * (kqb - pv[i * (n_b + 1) + 1 + kb]);
+    const double sqb = ys2 * (lnew[i * m + kb] - pv[i * (n_b + 1) + 1 + kb]);
     lnew[i * m + kb] = sqb;
     if (joint) dsqb[e] = sqb;
   }
