@@ -227,17 +227,24 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   if (has_x) xq[(tid / d) * MC_MAXD + tid % d] = xr;
   if (tid == 0) scal[1] = 0.0;
   __syncthreads();
-  // Kernel values K(x_i, x_j) into sqq and K(x_i, X_b) into lnew[:, :n_b]; they
+  // Kernel values K(x_i, x_j) into sqq and K(x_i, X_b) into lnew[:, :n_b], and
+  // with the gradient the radial factors of the direct input gradient into hh
+  // (Q x m: h(x_i, x_j) for j < Q, 0 on the diagonal, h(x_i, b_kb) after); they
   // need only the coordinates, so they run (from the last wave down) while the
   // V rows and the P partials are in flight.
+  double* hh = sm + lay.hh;
   for (int t = 255 - tid; t < Q * Q + Q * n_b; t += 256) {
     if (t < Q * Q) {
       const int i = t / Q, j = t % Q;
-      sqq[t] = kern_value(a.kind, sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d), a.outputscale);
+      const double r2 = sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d);
+      sqq[t] = kern_value(a.kind, r2, a.outputscale);
+      if (a.grad)
+        hh[i * m + j] = j == i ? 0.0 : kern_grad_factor(a.kind, r2, a.outputscale, nullptr);
     } else {
       const int e = t - Q * Q, i = e / n_b, kb = e % n_b;
-      lnew[i * m + kb] =
-          kern_value(a.kind, sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d), a.outputscale);
+      const double r2 = sq_dist(xq + i * MC_MAXD, a.bs + kb * d, d);
+      lnew[i * m + kb] = kern_value(a.kind, r2, a.outputscale);
+      if (a.grad) hh[i * m + Q + kb] = kern_grad_factor(a.kind, r2, a.outputscale, nullptr);
     }
   }
 
@@ -408,6 +415,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   typedef TV TM;
   const double tr = a.tau_relu, tm = a.tau_max, rtr = 1.0 / tr, rtm = 1.0 / tm;
   bool bad = false;
+  double lmax = -INFINITY;  // this thread's max of u over its samples
   // y -> log improvement -> fat-max -> u[s], and the per-sample adjoint G[s]
   auto sample_tail = [&](const int s, const double bf, const double(&y)[Q]) {
     double v[Q], dv[Q];
@@ -429,7 +437,9 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
       sdp += dp[i];
       nties += (v[i] == M);
     }
-    uu[s] = M + tm * (double)log(P);
+    const double u = M + tm * (double)log(P);
+    uu[s] = u;
+    lmax = fmax(lmax, u);
     const TM cm = div_tm((TM)1 + div_tm(sdp, P), (TM)nties);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -471,11 +481,9 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
     if (two) sample_tail(s0 + 256, bf1, y1);
   }
   if (bad) scal[1] = 1.0;  // every writer stores the same value
-  __syncthreads();
 
-  // ---- logmeanexp over samples --------------------------------------------
-  double lmax = -INFINITY;
-  for (int s = tid; s < S; s += 256) lmax = fmax(lmax, uu[s]);
+  // ---- logmeanexp over samples (each thread reads back only its own u[s]
+  // and G[s] until the next barrier) ----------------------------------------
   const double Mu = block_max1(lmax, red);
   double lsum[1] = {0.0};
   for (int s = tid; s < S; s += 256) {  // uu keeps exp(u - max) for the adjoint
@@ -643,17 +651,8 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
     const int i = e / (n_b + 1), k = e % (n_b + 1);
     a.dext[(r0 + i) * (n_b + 1) + k] = k == 0 ? ys * dmu[i] : -ys2 * dsqb[i * n_b + k - 1];
   }
-  // ---- direct input gradient through K(X, X) and K(X, X_b) (scaled) ------
-  // radial factors first (one thread each), then the d coordinates
-  double* hh = sm + lay.hh;  // Q x m: h(x_i, x_i2) for i2 < Q, h(x_i, b_kb) after
-  for (int e = tid; e < Q * m; e += 256) {
-    const int i = e / m, j2 = e % m;
-    const double* xi = xq + i * MC_MAXD;
-    const double* x2 = j2 < Q ? xq + j2 * MC_MAXD : a.bs + (j2 - Q) * d;
-    hh[e] = j2 == i ? 0.0
-                    : kern_grad_factor(a.kind, sq_dist(xi, x2, d), a.outputscale, nullptr);
-  }
-  __syncthreads();
+  // ---- direct input gradient through K(X, X) and K(X, X_b) (scaled), from
+  // the radial factors hh computed with the kernel values ------------------
   for (int e = tid; e < Q * d; e += 256) {
     const int i = e / d, j = e % d;
     const double xij = xq[i * MC_MAXD + j];
