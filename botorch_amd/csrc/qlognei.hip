@@ -10,7 +10,8 @@
 //   qstep_grad    dX from U, gq, dext and dK/dxs (qstep.hip)
 //
 // Everything between the products and the input gradient is small and done
-// by one 256-thread workgroup per restart b in LDS (qlognei_mc_kernel):
+// in LDS by one 256-thread workgroup per restart b (qlognei_mc_kernel), or per
+// two restarts, two waves each, for the mixed modes' FP32 sample arithmetic:
 //
 //   Sigma_qq = s^2 (K(X,X) - V V^T)        Sigma_qb = s^2 (K(X,X_b) - K_x A)
 //   mu       = y_mean + s (c + K_x alpha)                       (s = y_std)
@@ -39,25 +40,25 @@
 
 namespace bta {
 
-template <int Q, typename TV>
+template <int Q, typename TV, int NR = 1>
 __global__ __launch_bounds__(256) void qlognei_mc_kernel(McArgs a) {
   extern __shared__ double sm_dyn[];
-  mc_body<Q, TV>(a, blockIdx.x, sm_dyn);
+  mc_body<Q, TV, NR>(a, blockIdx.x, sm_dyn);
 }
 
 // One launch for the adjoint GEMM U = V L^-1 (workgroups 0 .. W-1, split-FP16,
-// 64-column tiles) and the MC stage (workgroups W .. W+b-1, one per restart):
-// both need only V, and the MC workgroups fill the CUs as the GEMM's light
-// tiles finish, with no hand-off between queues.  The MC stage uses the first
-// McLayout bytes of the GEMM's LDS ring.
+// 64-column tiles) and the MC stage (workgroups W and on, NR restarts each):
+// both need only V, and the MC workgroups take the CU slots the GEMM workers
+// leave free, with no hand-off between queues.  The MC stage uses the first
+// NR McLayout areas of the GEMM's LDS ring.
 
-template <int Q, typename TV>
+template <int Q, typename TV, int NR>
 __global__ __launch_bounds__(256, 2) void adjoint_mc_kernel(H2Args g, int W, McArgs a) {
   __shared__ __attribute__((aligned(1024))) unsigned char lds[FUSED_LDS];
   if ((int)blockIdx.x < W)
     gemm_h2_body<64, false>(g, blockIdx.x, W, lds);
   else
-    mc_body<Q, TV>(a, (int)blockIdx.x - W, (double*)lds);
+    mc_body<Q, TV, NR>(a, (int)blockIdx.x - W, (double*)lds);
 }
 
 // BTA_SEQUENTIAL_ADJOINT=1 in the environment makes every call take the
@@ -67,6 +68,25 @@ __global__ __launch_bounds__(256, 2) void adjoint_mc_kernel(H2Args g, int W, McA
 static bool fused_disabled() {
   const char* e = getenv("BTA_SEQUENTIAL_ADJOINT");
   return e && e[0] == '1';
+}
+
+// Two restarts per MC workgroup (two waves each) in the split-FP16 mixed mode
+// with q <= 5, whose MC workgroups share the CUs with the adjoint GEMM's
+// workers in one launch (one round of paired workgroups in the slots the
+// workers leave free is shorter than two rounds of single ones), when both
+// layouts fit MC_LDS_MAX and the cached root is in use (not the joint mode).
+// Every launch of that mode's MC stage (fused or alone, value-only or with the
+// gradient) takes the same choice, and a restart's results depend only on it,
+// not on the partner restart, so its value-only and value+grad steps, chunked
+// or whole, keep giving identical values.  Run alone over the whole GPU
+// (mixed_f32), single workgroups fill both slots of every CU and stay.
+// BTA_MC_PAIR=0 in the environment keeps one restart per workgroup (how the
+// two are compared in the tests).
+static bool mc_paired(const McArgs& a, int q, bool f32, bool split) {
+  const char* e = getenv("BTA_MC_PAIR");
+  const McLayout lay(q, a.n_b, a.S);
+  return f32 && split && q <= 5 && !a.Lb_levels && !(e && e[0] == '0') &&
+         (size_t)2 * lay.total * sizeof(double) <= MC_LDS_MAX;
 }
 
 // Launches the fused kernel when it applies (q <= 5, mixed sample arithmetic,
@@ -83,20 +103,31 @@ static int launch_adjoint_mc(const H2Args& args, const McArgs& a, int q, int64_t
   if (!f32 || q > 5 || bn != 64 || g.seg_chunks > 0 || fused_disabled() ||
       (size_t)lay.total * sizeof(double) > MC_LDS_MAX || b < 1 || b > (1 << 24))
     return -1;
-  const dim3 grid((unsigned)(W + b));
-  switch (q) {
-    case 1: hipLaunchKernelGGL((adjoint_mc_kernel<1, float>), grid, dim3(256), 0, s, g, W, a); break;
-    case 2: hipLaunchKernelGGL((adjoint_mc_kernel<2, float>), grid, dim3(256), 0, s, g, W, a); break;
-    case 3: hipLaunchKernelGGL((adjoint_mc_kernel<3, float>), grid, dim3(256), 0, s, g, W, a); break;
-    case 4: hipLaunchKernelGGL((adjoint_mc_kernel<4, float>), grid, dim3(256), 0, s, g, W, a); break;
-    default: hipLaunchKernelGGL((adjoint_mc_kernel<5, float>), grid, dim3(256), 0, s, g, W, a); break;
+  if (mc_paired(a, q, f32, true)) {
+    const dim3 grid((unsigned)(W + (b + 1) / 2));
+    switch (q) {
+      case 1: hipLaunchKernelGGL((adjoint_mc_kernel<1, float, 2>), grid, dim3(256), 0, s, g, W, a); break;
+      case 2: hipLaunchKernelGGL((adjoint_mc_kernel<2, float, 2>), grid, dim3(256), 0, s, g, W, a); break;
+      case 3: hipLaunchKernelGGL((adjoint_mc_kernel<3, float, 2>), grid, dim3(256), 0, s, g, W, a); break;
+      case 4: hipLaunchKernelGGL((adjoint_mc_kernel<4, float, 2>), grid, dim3(256), 0, s, g, W, a); break;
+      default: hipLaunchKernelGGL((adjoint_mc_kernel<5, float, 2>), grid, dim3(256), 0, s, g, W, a); break;
+    }
+  } else {
+    const dim3 grid((unsigned)(W + b));
+    switch (q) {
+      case 1: hipLaunchKernelGGL((adjoint_mc_kernel<1, float, 1>), grid, dim3(256), 0, s, g, W, a); break;
+      case 2: hipLaunchKernelGGL((adjoint_mc_kernel<2, float, 1>), grid, dim3(256), 0, s, g, W, a); break;
+      case 3: hipLaunchKernelGGL((adjoint_mc_kernel<3, float, 1>), grid, dim3(256), 0, s, g, W, a); break;
+      case 4: hipLaunchKernelGGL((adjoint_mc_kernel<4, float, 1>), grid, dim3(256), 0, s, g, W, a); break;
+      default: hipLaunchKernelGGL((adjoint_mc_kernel<5, float, 1>), grid, dim3(256), 0, s, g, W, a); break;
+    }
   }
   BTA_LAUNCH_CHECK("adjoint_mc_kernel");
   return BTA_OK;
 }
 
 template <int Q>
-static int launch_mc_q(const McArgs& a, int64_t b, bool f32, hipStream_t s) {
+static int launch_mc_q(const McArgs& a, int64_t b, bool f32, bool split, hipStream_t s) {
   const McLayout lay(Q, a.n_b, a.S);
   const size_t bytes = (size_t)lay.total * sizeof(double);
   if (bytes > MC_LDS_MAX) {
@@ -104,7 +135,10 @@ static int launch_mc_q(const McArgs& a, int64_t b, bool f32, hipStream_t s) {
               a.S);
     return BTA_ERR_UNSUPPORTED;
   }
-  if (f32)
+  if (mc_paired(a, Q, f32, split))
+    hipLaunchKernelGGL((qlognei_mc_kernel<Q, float, 2>), dim3((unsigned)((b + 1) / 2)), dim3(256),
+                       2 * bytes, s, a);
+  else if (f32)
     hipLaunchKernelGGL((qlognei_mc_kernel<Q, float>), dim3((unsigned)b), dim3(256), bytes, s, a);
   else
     hipLaunchKernelGGL((qlognei_mc_kernel<Q, double>), dim3((unsigned)b), dim3(256), bytes, s,
@@ -113,16 +147,16 @@ static int launch_mc_q(const McArgs& a, int64_t b, bool f32, hipStream_t s) {
   return BTA_OK;
 }
 
-static int launch_mc(const McArgs& a, int q, int64_t b, bool f32, hipStream_t s) {
+static int launch_mc(const McArgs& a, int q, int64_t b, bool f32, bool split, hipStream_t s) {
   switch (q) {
-    case 1: return launch_mc_q<1>(a, b, f32, s);
-    case 2: return launch_mc_q<2>(a, b, f32, s);
-    case 3: return launch_mc_q<3>(a, b, f32, s);
-    case 4: return launch_mc_q<4>(a, b, f32, s);
-    case 5: return launch_mc_q<5>(a, b, f32, s);
-    case 6: return launch_mc_q<6>(a, b, f32, s);
-    case 7: return launch_mc_q<7>(a, b, f32, s);
-    case 8: return launch_mc_q<8>(a, b, f32, s);
+    case 1: return launch_mc_q<1>(a, b, f32, split, s);
+    case 2: return launch_mc_q<2>(a, b, f32, split, s);
+    case 3: return launch_mc_q<3>(a, b, f32, split, s);
+    case 4: return launch_mc_q<4>(a, b, f32, split, s);
+    case 5: return launch_mc_q<5>(a, b, f32, split, s);
+    case 6: return launch_mc_q<6>(a, b, f32, split, s);
+    case 7: return launch_mc_q<7>(a, b, f32, split, s);
+    case 8: return launch_mc_q<8>(a, b, f32, split, s);
   }
   set_error("qlognei: q must be in 1..8 (got %d)", q);
   return BTA_ERR_UNSUPPORTED;
@@ -299,6 +333,7 @@ int qlognei_step_args(const bta_gp_model* m, const bta_qlognei_state* st, const 
   a.n_pad = (int)np;
   a.n_b = st->n_b;
   a.S = st->S;
+  a.nrest = (int)b;
   a.kind = m->kind;
   a.max_tries = st->max_tries;
   a.grad = grad != nullptr;
@@ -367,7 +402,7 @@ int qlognei_value_grad(const bta_gp_model* m, const bta_qlognei_state* st, const
       rc = gemm_h2(A.adj, s);
       if (rc) return rc;
     }
-    rc = launch_mc(A.mc, A.q, A.b, A.f32, s);
+    rc = launch_mc(A.mc, A.q, A.b, A.f32, A.split, s);
   }
   if (rc || !A.grad) return rc;
   return qstep_grad(A.g, A.f32, A.fin.extra, A.fin.ls, A.fin.dX, s);

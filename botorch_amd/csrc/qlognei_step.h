@@ -36,6 +36,7 @@ struct McArgs {
   double* dxs_extra;
   int* status;
   int d, n_pad, n_b, S, kind, max_tries, grad;
+  int nrest;  // restarts (b)
   double outputscale, y_mean, y_std, c, tau_relu, tau_max, jitter;
 };
 
@@ -157,6 +158,47 @@ __device__ __forceinline__ double block_max1(double v, double* red) {
   return r;
 }
 
+// Reductions over the WPR waves of one restart (wave w of the restart at
+// red[w * 64 ...]): the sums take the waves in order, as block_sum does.
+template <int NV, int WPR>
+__device__ __forceinline__ void group_sum(double (&v)[NV], double* red, double (&out)[NV], int wv) {
+  if (NV <= 32) {
+    int e;
+    const double s =
+        wave_sum_spread<(NV <= 32 ? NV : 1)>(*(const double(*)[NV <= 32 ? NV : 1])v, &e);
+    if (e >= 0) red[wv * 64 + e] = s;
+  } else {
+    double s[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] = v[i];
+    wave_sum_n<NV>(s);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int i = 0; i < NV; ++i) red[wv * 64 + i] = s[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double t = red[i];
+#pragma unroll
+    for (int w = 1; w < WPR; ++w) t += red[w * 64 + i];
+    out[i] = t;
+  }
+  __syncthreads();
+}
+
+template <int WPR>
+__device__ __forceinline__ double group_max1(double v, double* red, int wv) {
+  const double s = wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[wv * 64] = s;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < WPR; ++w) r = fmax(r, red[w * 64]);
+  __syncthreads();
+  return r;
+}
+
 __device__ __forceinline__ double sq_dist(const double* a, const double* b, int d) {
   double r2 = 0.0;
   for (int j = 0; j < d; ++j) {
@@ -185,11 +227,22 @@ __device__ bool small_chol(const double* A, double jit, double* L) {
   return true;
 }
 
-// Restart b's MC stage and its adjoint, with sm the McLayout area in LDS.
-template <int Q, typename TV>
-__device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm) {
+// The MC stage and its adjoint for NR restarts per workgroup (NR = 1 or 2):
+// restart bw * NR + h runs on threads [h TPR, (h + 1) TPR) with the McLayout
+// area sm_all + h * total in LDS.  With NR = 2 the two restarts pass the same
+// barriers, so their serial phases overlap; a half past the last restart
+// repeats the last one and stores nothing.
+template <int Q, typename TV, int NR = 1>
+__device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* sm_all) {
+  static_assert(NR == 1 || NR == 2, "one or two restarts per workgroup");
+  constexpr int TPR = 256 / NR, WPR = 4 / NR;  // threads and waves per restart
   const McLayout lay(Q, a.n_b, a.S);
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int half = (int)threadIdx.x / TPR;
+  const int tid = (int)threadIdx.x % TPR, lane = tid & 63, wv = tid >> 6;
+  const int bt = bw * NR + half;
+  const bool live = bt < a.nrest;
+  const int b = live ? bt : a.nrest - 1;
+  double* sm = sm_all + half * lay.total;
   const int d = a.d, n_b = a.n_b, S = a.S, m = n_b + Q;
   const double ys = a.y_std, ys2 = a.y_std * a.y_std;
   double* xq = sm + lay.xq;
@@ -211,7 +264,6 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   // the query coordinates (one per thread), then the restart's V rows for
   // V_b V_b^T (16-byte loads): their first two column passes are issued here,
   // ahead of the other loads of the stage
-  static_assert(Q * MC_MAXD <= 256, "one query coordinate per thread");
   const bool has_x = tid < Q * d;
   const double xr = has_x ? a.xs[r0 * d + tid] : 0.0;
   constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
@@ -219,12 +271,13 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   vec_t vpre[2][Q];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    const int k = (tid + 256 * h) * VW;
+    const int k = (tid + TPR * h) * VW;
 #pragma unroll
     for (int i = 0; i < Q; ++i)
       vpre[h][i] = k < a.n_pad ? *(const vec_t*)(vt + (r0 + i) * a.ldv + k) : vec_t{};
   }
   if (has_x) xq[(tid / d) * MC_MAXD + tid % d] = xr;
+  for (int i = tid + TPR; i < Q * d; i += TPR) xq[(i / d) * MC_MAXD + i % d] = a.xs[r0 * d + i];
   if (tid == 0) scal[1] = 0.0;
   __syncthreads();
   // Kernel values K(x_i, x_j) into sqq and K(x_i, X_b) into lnew[:, :n_b], and
@@ -233,7 +286,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   // need only the coordinates, so they run (from the last wave down) while the
   // V rows and the P partials are in flight.
   double* hh = sm + lay.hh;
-  for (int t = 255 - tid; t < Q * Q + Q * n_b; t += 256) {
+  for (int t = TPR - 1 - tid; t < Q * Q + Q * n_b; t += TPR) {
     if (t < Q * Q) {
       const int i = t / Q, j = t % Q;
       const double r2 = sq_dist(xq + i * MC_MAXD, xq + j * MC_MAXD, d);
@@ -257,7 +310,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   {
     // four tasks per wave per pass, their loads issued together
     const int ne = n_b + 1;
-    for (int t0 = wv * 4; t0 < Q * ne; t0 += 16) {
+    for (int t0 = wv * 4; t0 < Q * ne; t0 += 4 * WPR) {
       double s[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -290,15 +343,15 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
     };
 #pragma unroll
     for (int h = 0; h < 2; ++h)
-      if ((tid + 256 * h) * VW < a.n_pad) vv_pass(vpre[h]);
-    for (int k = (tid + 512) * VW; k < a.n_pad; k += 256 * VW) {
+      if ((tid + TPR * h) * VW < a.n_pad) vv_pass(vpre[h]);
+    for (int k = (tid + 2 * TPR) * VW; k < a.n_pad; k += TPR * VW) {
       vec_t v[Q];
 #pragma unroll
       for (int i = 0; i < Q; ++i) v[i] = *(const vec_t*)(vt + (r0 + i) * a.ldv + k);
       vv_pass(v);
     }
     double tot[NQ];
-    block_sum<NQ>(acc, red, tot);
+    group_sum<NQ, WPR>(acc, red, tot, wv);
     if (tid < NQ) vv[tid] = tot[tid];
   }
   __syncthreads();
@@ -310,7 +363,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + pv[tid * (n_b + 1)]);
   // ---- Sigma_qb into lnew[:, :n_b] (and a copy in dsqb for the joint mode)
   const bool joint = a.Lb_levels != nullptr;
-  for (int e = tid; e < Q * n_b; e += 256) {
+  for (int e = tid; e < Q * n_b; e += TPR) {
     const int i = e / n_b, kb = e % n_b;
     const double sqb = ys2 * (lnew[i * m + kb] - pv[i * (n_b + 1) + 1 + kb]);
     lnew[i * m + kb] = sqb;
@@ -331,12 +384,12 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
       if (!((a.lb_levels_mask >> t) & 1)) continue;  // the baseline block fails too
       Lb = a.Lb_levels + (int64_t)t * n_b * n_b;
       if (t > 0) {
-        for (int e = tid; e < Q * n_b; e += 256) lnew[(e / n_b) * m + e % n_b] = dsqb[e];
+        for (int e = tid; e < Q * n_b; e += TPR) lnew[(e / n_b) * m + e % n_b] = dsqb[e];
         __syncthreads();
       }
     }
     // ---- Lbl = Sigma_qb Lb^-T: forward substitution, one wave per row ----
-    for (int i = wv; i < Q; i += 4) {
+    for (int i = wv; i < Q; i += WPR) {
       double rhs[MC_MAXNB_SLOTS];
 #pragma unroll
       for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
@@ -403,7 +456,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
     for (int i = 0; i < Q * Q; ++i) lbr[i] = NAN;
   }
   __syncthreads();
-  for (int e = tid; e < Q * Q; e += 256) {
+  for (int e = tid; e < Q * Q; e += TPR) {
     const int i = e / Q, j = e % Q;
     lnew[i * m + n_b + j] = lbr[i * Q + j];
   }
@@ -447,13 +500,13 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
       G[s * Q + i] = (double)du * dv[i];
     }
   };
-  // Each thread carries the samples s0 and s0 + 256 through the products: the
+  // Each thread carries the samples s0 and s0 + TPR through the products: the
   // rows of [Lbl | Lbr] (LDS) and the base samples are read eight columns at a
   // time at clamped indices, so that every read issues before the products;
   // the products skip the columns past m.
-  for (int s0 = tid; s0 < S; s0 += 512) {
-    const bool two = s0 + 256 < S;
-    const int s1 = two ? s0 + 256 : s0;
+  for (int s0 = tid; s0 < S; s0 += 2 * TPR) {
+    const bool two = s0 + TPR < S;
+    const int s1 = two ? s0 + TPR : s0;
     const double bf0 = a.best_f[s0], bf1 = a.best_f[s1];
     double y0[Q], y1[Q];
 #pragma unroll
@@ -478,22 +531,22 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
           }
     }
     sample_tail(s0, bf0, y0);
-    if (two) sample_tail(s0 + 256, bf1, y1);
+    if (two) sample_tail(s0 + TPR, bf1, y1);
   }
   if (bad) scal[1] = 1.0;  // every writer stores the same value
 
   // ---- logmeanexp over samples (each thread reads back only its own u[s]
   // and G[s] until the next barrier) ----------------------------------------
-  const double Mu = block_max1(lmax, red);
+  const double Mu = group_max1<WPR>(lmax, red, wv);
   double lsum[1] = {0.0};
-  for (int s = tid; s < S; s += 256) {  // uu keeps exp(u - max) for the adjoint
+  for (int s = tid; s < S; s += TPR) {  // uu keeps exp(u - max) for the adjoint
     const double e = exp(uu[s] - Mu);
     uu[s] = e;
     lsum[0] += e;
   }
   double tsum[1];
-  block_sum<1>(lsum, red, tsum);
-  if (tid == 0) {
+  group_sum<1, WPR>(lsum, red, tsum, wv);
+  if (tid == 0 && live) {
     a.value[b] = Mu + log(tsum[0]) - log((double)S);
     int st = (int)scal[0];
     if (scal[1] != 0.0) st |= BTA_STATUS_NAN;
@@ -508,7 +561,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
     double acc[NV];
 #pragma unroll
     for (int e = 0; e < NV; ++e) acc[e] = 0.0;
-    for (int s = tid; s < S; s += 256) {
+    for (int s = tid; s < S; s += TPR) {
       double z[MREG];  // loaded ahead of the products (uniform branches)
 #pragma unroll
       for (int k = 0; k < MREG; ++k)
@@ -535,16 +588,18 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
       if (e >= 0) red[wv * 64 + e] = sw;
     }
     __syncthreads();
-    for (int e = tid; e < Q * (m + 1); e += 256) {
+    for (int e = tid; e < Q * (m + 1); e += TPR) {
       const int i = e / (m + 1), k = e % (m + 1);
       const int idx = i * (MREG + 1) + (k == m ? MREG : k);
-      const double tot = red[idx] + red[64 + idx] + red[128 + idx] + red[192 + idx];
+      double tot = red[idx];
+#pragma unroll
+      for (int w = 1; w < WPR; ++w) tot += red[w * 64 + idx];
       if (k == m) dmu[i] = tot;
       else dlnew[i * m + k] = k < n_b + i + 1 ? tot : 0.0;
     }
     __syncthreads();
   } else {
-  for (int s = tid; s < S; s += 256) {
+  for (int s = tid; s < S; s += TPR) {
     const double w = uu[s] / tsum[0];
 #pragma unroll
     for (int i = 0; i < Q; ++i) G[s * Q + i] *= w;
@@ -552,7 +607,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   __syncthreads();
 
   // one wave per (i, k) for wide baselines
-  for (int task = wv; task < Q * (m + 1); task += 4) {
+  for (int task = wv; task < Q * (m + 1); task += WPR) {
     const int i = task / (m + 1), k = task % (m + 1);
     if (k < m && k >= n_b + i + 1) {  // upper part of Lbr: no dependence
       if (lane == 0) dlnew[i * m + k] = 0.0;
@@ -607,7 +662,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   }
   __syncthreads();
   // dLbl -= 2 gA Lbl  (Schur = Sigma_qq - Lbl Lbl^T)
-  for (int e = tid; e < Q * n_b; e += 256) {
+  for (int e = tid; e < Q * n_b; e += TPR) {
     const int i = e / n_b, kb = e % n_b;
     double s = 0.0;
 #pragma unroll
@@ -616,7 +671,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   }
   __syncthreads();
   // ---- dSigma_qb = dLbl Lb^-1 : back substitution, one wave per row -------
-  for (int i = wv; i < Q; i += 4) {
+  for (int i = wv; i < Q; i += WPR) {
     double rhs[MC_MAXNB_SLOTS];
 #pragma unroll
     for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
@@ -646,14 +701,15 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
   __syncthreads();
 
   // ---- dV = gq V with gq = -2 s^2 gA: the q x q block for the gradient stage
-  for (int e = tid; e < Q * Q; e += 256) a.gq[r0 * Q + e] = -2.0 * ys2 * gA[e];
-  for (int e = tid; e < Q * (n_b + 1); e += 256) {
+  if (!live) return;
+  for (int e = tid; e < Q * Q; e += TPR) a.gq[r0 * Q + e] = -2.0 * ys2 * gA[e];
+  for (int e = tid; e < Q * (n_b + 1); e += TPR) {
     const int i = e / (n_b + 1), k = e % (n_b + 1);
     a.dext[(r0 + i) * (n_b + 1) + k] = k == 0 ? ys * dmu[i] : -ys2 * dsqb[i * n_b + k - 1];
   }
   // ---- direct input gradient through K(X, X) and K(X, X_b) (scaled), from
   // the radial factors hh computed with the kernel values ------------------
-  for (int e = tid; e < Q * d; e += 256) {
+  for (int e = tid; e < Q * d; e += TPR) {
     const int i = e / d, j = e % d;
     const double xij = xq[i * MC_MAXD + j];
     double acc = 0.0;
@@ -666,7 +722,8 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int b, double* sm
 }
 
 // One launch for the adjoint GEMM U = V L^-1 and the MC stage uses the first
-// McLayout bytes of the GEMM's LDS ring (64-column tiles).
+// NR McLayout areas of the GEMM's LDS ring (64-column tiles; NR areas stay
+// within MC_LDS_MAX).
 constexpr int FUSED_LDS = H2Cfg<64, false>::STAGES * H2Cfg<64, false>::STAGE;
 static_assert(MC_LDS_MAX <= (size_t)FUSED_LDS, "MC layout must fit the fused ring");
 

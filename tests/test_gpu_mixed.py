@@ -167,23 +167,55 @@ def test_qlognei_step_graph_replay(precision):
         assert torch.equal(v_g, v_e) and torch.equal(g_g, g_e) and torch.equal(st_g, st_e)
 
 
+def _launch_env(acq, X, env):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        out = acq._launch(X, True)
+        torch.cuda.synchronize()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    return out
+
+
 def test_fused_adjoint_equals_sequential_path():
     """adjoint_mc_kernel runs the adjoint GEMM body and the MC body of the
-    sequential path (gemm_h2_kernel, then qlognei_mc_kernel) on the same V: the
-    two paths give identical values, gradients and status words."""
-    import os
+    sequential path (gemm_h2_kernel, then qlognei_mc_kernel) on the same V: with
+    one restart per MC workgroup, and with two, the two paths give identical
+    values, gradients and status words."""
     _say("fused vs sequential adjoint n=2048 d=12 q=4 S=512 b=128 mixed")
     acq, _, Xq = _acq_pair(2048, 12, 4, 512, "mixed", seed=0)
     Xd = Xq.to(DEV).contiguous()
-    v_f, g_f, s_f = acq._launch(Xd, True)
-    torch.cuda.synchronize()
-    os.environ["BTA_SEQUENTIAL_ADJOINT"] = "1"
-    try:
-        v_s, g_s, s_s = acq._launch(Xd, True)
-        torch.cuda.synchronize()
-    finally:
-        del os.environ["BTA_SEQUENTIAL_ADJOINT"]
-    assert torch.equal(v_f, v_s) and torch.equal(g_f, g_s) and torch.equal(s_f, s_s)
+    for pair in ("0", "1"):
+        v_f, g_f, s_f = _launch_env(acq, Xd, {"BTA_MC_PAIR": pair})
+        v_s, g_s, s_s = _launch_env(acq, Xd, {"BTA_MC_PAIR": pair, "BTA_SEQUENTIAL_ADJOINT": "1"})
+        assert torch.equal(v_f, v_s) and torch.equal(g_f, g_s) and torch.equal(s_f, s_s), pair
+
+
+@pytest.mark.parametrize("q,b", [(4, 127), (2, 128), (5, 33), (1, 1)])
+def test_fused_paired_restarts_match_single(q, b):
+    """The fused launch runs two restarts per MC workgroup (two waves each): each
+    restart's reductions then run over two waves instead of four, so values and
+    gradients differ from the one-restart launch by rounding only, and the
+    status words agree.  An odd restart count leaves the second half of the last
+    workgroup without a restart of its own."""
+    _say(f"paired vs single MC restarts n=1024 d=6 q={q} S=512 b={b} mixed")
+    acq, ref, Xq = _acq_pair(1024, 6, q, 512, "mixed", seed=5)
+    Xd = Xq[:b].to(DEV).contiguous()
+    v_p, g_p, s_p = _launch_env(acq, Xd, {"BTA_MC_PAIR": "1"})
+    v_1, g_1, s_1 = _launch_env(acq, Xd, {"BTA_MC_PAIR": "0"})
+    assert v_p.shape == (b,) and g_p.shape == Xd.shape
+    assert torch.equal(s_p, s_1)
+    assert ((v_p - v_1).abs().max() / v_1.abs().max()).item() < 1e-9
+    assert ((g_p - g_1).abs().max() / g_1.abs().max()).item() < 1e-9
+    with torch.no_grad():
+        v_ref = ref(Xq[:b])
+    assert ((v_p.cpu() - v_ref).abs().max() / v_ref.abs().max()).item() < 1e-4
 
 
 @pytest.mark.parametrize("S,fits", [(3700, True), (4096, False)])
