@@ -308,12 +308,14 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
   double* pv = sm + lay.pv;  // Q x (n_b + 1): P[r0 + i][j]
   double* vv = sm + lay.vv;  // packed lower triangle of V_b V_b^T
   {
-    // four tasks per wave per pass, their loads issued together
+    // 4 NR tasks per wave per pass (16 per restart), their loads issued
+    // together; each task's sum is the same whatever the batch
+    constexpr int TB = 4 * NR;
     const int ne = n_b + 1;
-    for (int t0 = wv * 4; t0 < Q * ne; t0 += 4 * WPR) {
-      double s[4];
+    for (int t0 = wv * TB; t0 < Q * ne; t0 += TB * WPR) {
+      double s[TB];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < TB; ++u) {
         const int t = t0 + u;
         s[u] = 0.0;
         if (t < Q * ne) {
@@ -322,7 +324,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
         }
       }
       int e;
-      const double tot = wave_sum_spread<4>(s, &e);
+      const double tot = wave_sum_spread<TB>(s, &e);
       if (e >= 0 && t0 + e < Q * ne) pv[t0 + e] = tot;
     }
     double acc[NQ];
@@ -561,11 +563,7 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
     double acc[NV];
 #pragma unroll
     for (int e = 0; e < NV; ++e) acc[e] = 0.0;
-    for (int s = tid; s < S; s += TPR) {
-      double z[MREG];  // loaded ahead of the products (uniform branches)
-#pragma unroll
-      for (int k = 0; k < MREG; ++k)
-        if (k < m) z[k] = a.Zt[(int64_t)k * S + s];
+    auto accum = [&](const int s, const double(&z)[MREG]) {
       const double w = uu[s] / tsum[0];
       double gw[Q];
 #pragma unroll
@@ -581,6 +579,21 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
             acc[i * (MREG + 1) + k] = fma(gw[i], z[k], acc[i * (MREG + 1) + k]);
         }
       }
+    };
+    // samples s and s + TPR per pass, the base samples of both loaded ahead of
+    // the products (uniform branches); the sums run in sample order as before
+    for (int s = tid; s < S; s += 2 * TPR) {
+      const bool two = s + TPR < S;
+      const int s2 = two ? s + TPR : s;
+      double z[MREG], z2[MREG];
+#pragma unroll
+      for (int k = 0; k < MREG; ++k)
+        if (k < m) {
+          z[k] = a.Zt[(int64_t)k * S + s];
+          z2[k] = a.Zt[(int64_t)k * S + s2];
+        }
+      accum(s, z);
+      if (two) accum(s2, z2);
     }
     {
       int e;
