@@ -101,7 +101,8 @@ struct QsCovarArgs {
   void* K;                // R_pad x ldk (FP64 / FP32 / FP16 hi plane)
   _Float16* Klo;          // FP16 lo plane (mode 2)
   double* xs_out;         // R_pad x d scaled rows (zero beyond R)
-  double* Ppart;          // (n_pad / 64) x R x ne partial products
+  double* Ppart;          // R x ne x (n_pad / 64) partial products: one (row, column)'s
+                          // column-tile partials are contiguous
   int64_t R, R_pad, n, n_pad, ldk;
   int d, kind, ne;
   double s;

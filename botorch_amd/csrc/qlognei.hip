@@ -324,7 +324,6 @@ int qlognei_step_args(const bta_gp_model* m, const bta_qlognei_state* st, const 
   a.best_f = st->best_f;
   a.P = P;
   a.p_parts = w.parts;
-  a.p_stride = w.R * w.ne;
   a.dext = dext;
   a.value = value;
   a.dxs_extra = dxe;

@@ -28,8 +28,7 @@ struct McArgs {
   int lb_levels_mask;       // bit t: Lb_levels[t] exists
   const double* Zt;
   const double* best_f;
-  const double* P;    // p_parts x (R x ne) partial sums of K_x [alpha | A] (FP64)
-  int64_t p_stride;   // elements between partials
+  const double* P;    // R x ne x p_parts partial sums of K_x [alpha | A] (FP64)
   int p_parts;
   double* dext;       // R x ne: adjoint of P
   double* value;
@@ -319,8 +318,8 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
         const int t = t0 + u;
         s[u] = 0.0;
         if (t < Q * ne) {
-          const double* p = a.P + (r0 + t / ne) * ne + t % ne;
-          for (int pt = lane; pt < a.p_parts; pt += 64) s[u] += p[pt * a.p_stride];
+          const double* p = a.P + ((r0 + t / ne) * ne + t % ne) * a.p_parts;
+          for (int pt = lane; pt < a.p_parts; pt += 64) s[u] += p[pt];
         }
       }
       int e;

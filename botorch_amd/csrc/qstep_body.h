@@ -145,7 +145,8 @@ __device__ __forceinline__ void qstep_covar_body(const QsCovarArgs& a, const int
       if (MODE == 2) *(int4*)(a.Klo + o) = *(const int4*)&ks[MODE == 2 ? 1 : 0].k[r][cc];
     }
   }
-  // Ppart[ct][r][j] = sum_c K[r][c] E[c][j] in chunks of 16 columns j
+  // Ppart[r][j][ct] = sum_c K[r][c] E[c][j] in chunks of 16 columns j
+  const int64_t nct = a.n_pad / QS_CT;
   for (int j0 = 0; j0 < a.ne; j0 += 16) {
     const int jj = j0 + l16;
     v4d p = {0.0, 0.0, 0.0, 0.0};
@@ -162,7 +163,7 @@ __device__ __forceinline__ void qstep_covar_body(const QsCovarArgs& a, const int
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t r = r0w + g + 4 * q;
-      if (jj < a.ne && r < a.R) a.Ppart[((int64_t)ct * a.R + r) * a.ne + jj] = p[q];
+      if (jj < a.ne && r < a.R) a.Ppart[(r * a.ne + jj) * nct + ct] = p[q];
     }
   }
 }
