@@ -12,6 +12,7 @@ namespace bta {
 
 constexpr int MC_MAXD = 32;
 constexpr int MC_MAXNB_SLOTS = 4;  // n_b <= 256
+constexpr int MC_LB_STAGE = 16;    // Lb staged in LDS for n_b up to this
 // LDS bound of the MC stage, the same for the standalone and the fused
 // launches so that a state is accepted by both or by neither (the value-only
 // call runs the standalone kernel, value+grad may run the fused one).
@@ -41,7 +42,7 @@ struct McArgs {
 
 // LDS layout (doubles) shared by host sizing and the kernel.
 struct McLayout {
-  int xq, red, mu, dmu, sqq, lbr, ga, lnew, dlnew, dsqb, u, G, scal, pv, vv, hh, total;
+  int xq, red, mu, dmu, sqq, lbr, ga, lnew, dlnew, dsqb, u, G, scal, pv, vv, hh, lb, lb_n, total;
   __host__ __device__ McLayout(int Q, int n_b, int S) {
     const int m = n_b + Q;
     int o = 0;
@@ -61,6 +62,12 @@ struct McLayout {
     u = o; o += S;
     G = o; o += S * Q;
     scal = o; o += 8;
+    // Lb (n_b x n_b) for the triangular solves, when n_b <= MC_LB_STAGE and it
+    // keeps the layout within MC_LDS_MAX
+    lb = o;
+    lb_n = n_b <= MC_LB_STAGE && (size_t)(o + n_b * n_b) * sizeof(double) <= MC_LDS_MAX
+               ? n_b * n_b : 0;
+    o += lb_n;
     total = o;
   }
 };
@@ -275,6 +282,13 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
     for (int i = 0; i < Q; ++i)
       vpre[h][i] = k < a.n_pad ? *(const vec_t*)(vt + (r0 + i) * a.ldv + k) : vec_t{};
   }
+  // In the cached-root mode Lb goes to LDS (lay.lb_n entries) for the two
+  // triangular solves, whose column loops would otherwise wait on a global load
+  // per step; the first TPR entries load here, with the V rows.
+  const bool joint = a.Lb_levels != nullptr;
+  const bool lb_lds = !joint && lay.lb_n > 0;
+  double* lbs = sm + lay.lb;
+  const double lbv = lb_lds && tid < lay.lb_n ? a.Lb[tid] : 0.0;
   if (has_x) xq[(tid / d) * MC_MAXD + tid % d] = xr;
   for (int i = tid + TPR; i < Q * d; i += TPR) xq[(i / d) * MC_MAXD + i % d] = a.xs[r0 * d + i];
   if (tid == 0) scal[1] = 0.0;
@@ -363,7 +377,10 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
   }
   if (tid < Q) mu[tid] = a.y_mean + ys * (a.c + pv[tid * (n_b + 1)]);
   // ---- Sigma_qb into lnew[:, :n_b] (and a copy in dsqb for the joint mode)
-  const bool joint = a.Lb_levels != nullptr;
+  if (lb_lds) {
+    if (tid < lay.lb_n) lbs[tid] = lbv;
+    for (int e = tid + TPR; e < lay.lb_n; e += TPR) lbs[e] = a.Lb[e];
+  }
   for (int e = tid; e < Q * n_b; e += TPR) {
     const int i = e / n_b, kb = e % n_b;
     const double sqb = ys2 * (lnew[i * m + kb] - pv[i * (n_b + 1) + 1 + kb]);
@@ -390,33 +407,37 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
       }
     }
     // ---- Lbl = Sigma_qb Lb^-T: forward substitution, one wave per row ----
-    for (int i = wv; i < Q; i += WPR) {
-      double rhs[MC_MAXNB_SLOTS];
-#pragma unroll
-      for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
-        const int j = lane + 64 * t2;
-        rhs[t2] = j < n_b ? lnew[i * m + j] : 0.0;
-      }
-      for (int kb = 0; kb < n_b; ++kb) {
-        const int slot = kb >> 6, owner = kb & 63;
-        double mine = 0.0;
-#pragma unroll
-        for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2)
-          if (t2 == slot) mine = rhs[t2];
-        const double y = __shfl(mine, owner, 64) / Lb[kb * n_b + kb];
+    auto lbl_rows = [&](const double* L) {
+      for (int i = wv; i < Q; i += WPR) {
+        double rhs[MC_MAXNB_SLOTS];
 #pragma unroll
         for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
           const int j = lane + 64 * t2;
-          if (j == kb) rhs[t2] = y;
-          else if (j > kb && j < n_b) rhs[t2] -= Lb[j * n_b + kb] * y;
+          rhs[t2] = j < n_b ? lnew[i * m + j] : 0.0;
+        }
+        for (int kb = 0; kb < n_b; ++kb) {
+          const int slot = kb >> 6, owner = kb & 63;
+          double mine = 0.0;
+#pragma unroll
+          for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2)
+            if (t2 == slot) mine = rhs[t2];
+          const double y = __shfl(mine, owner, 64) / L[kb * n_b + kb];
+#pragma unroll
+          for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
+            const int j = lane + 64 * t2;
+            if (j == kb) rhs[t2] = y;
+            else if (j > kb && j < n_b) rhs[t2] -= L[j * n_b + kb] * y;
+          }
+        }
+#pragma unroll
+        for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
+          const int j = lane + 64 * t2;
+          if (j < n_b) lnew[i * m + j] = rhs[t2];
         }
       }
-#pragma unroll
-      for (int t2 = 0; t2 < MC_MAXNB_SLOTS; ++t2) {
-        const int j = lane + 64 * t2;
-        if (j < n_b) lnew[i * m + j] = rhs[t2];
-      }
-    }
+    };
+    if (lb_lds) lbl_rows(lbs);
+    else lbl_rows(Lb);
     __syncthreads();
 
     // ---- Schur complement and its Cholesky --------------------------------
@@ -683,33 +704,37 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
   }
   __syncthreads();
   // ---- dSigma_qb = dLbl Lb^-1 : back substitution, one wave per row -------
-  for (int i = wv; i < Q; i += WPR) {
-    double rhs[MC_MAXNB_SLOTS];
-#pragma unroll
-    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
-      const int j = lane + 64 * t;
-      rhs[t] = j < n_b ? dlnew[i * m + j] : 0.0;
-    }
-    for (int kb = n_b - 1; kb >= 0; --kb) {
-      const int slot = kb >> 6, owner = kb & 63;
-      double mine = 0.0;
-#pragma unroll
-      for (int t = 0; t < MC_MAXNB_SLOTS; ++t)
-        if (t == slot) mine = rhs[t];
-      const double zz = __shfl(mine, owner, 64) / Lb[kb * n_b + kb];
+  auto dsqb_rows = [&](const double* L) {
+    for (int i = wv; i < Q; i += WPR) {
+      double rhs[MC_MAXNB_SLOTS];
 #pragma unroll
       for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
         const int j = lane + 64 * t;
-        if (j == kb) rhs[t] = zz;
-        else if (j < kb) rhs[t] -= Lb[kb * n_b + j] * zz;
+        rhs[t] = j < n_b ? dlnew[i * m + j] : 0.0;
+      }
+      for (int kb = n_b - 1; kb >= 0; --kb) {
+        const int slot = kb >> 6, owner = kb & 63;
+        double mine = 0.0;
+#pragma unroll
+        for (int t = 0; t < MC_MAXNB_SLOTS; ++t)
+          if (t == slot) mine = rhs[t];
+        const double zz = __shfl(mine, owner, 64) / L[kb * n_b + kb];
+#pragma unroll
+        for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+          const int j = lane + 64 * t;
+          if (j == kb) rhs[t] = zz;
+          else if (j < kb) rhs[t] -= L[kb * n_b + j] * zz;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
+        const int j = lane + 64 * t;
+        if (j < n_b) dsqb[i * n_b + j] = rhs[t];
       }
     }
-#pragma unroll
-    for (int t = 0; t < MC_MAXNB_SLOTS; ++t) {
-      const int j = lane + 64 * t;
-      if (j < n_b) dsqb[i * n_b + j] = rhs[t];
-    }
-  }
+  };
+  if (lb_lds) dsqb_rows(lbs);
+  else dsqb_rows(Lb);
   __syncthreads();
 
   // ---- dV = gq V with gq = -2 s^2 gA: the q x q block for the gradient stage
