@@ -114,58 +114,10 @@ __device__ __forceinline__ void log_fatplus_grad(double x, double tau, double rt
   *dv = (double)(f32 ? g * (TM)rtau : g / tt);
 }
 
-// block-wide reduction of NV values (NV <= 64); every thread gets the result
-// in out[]; uses red (4*64 doubles).  All threads must call.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* red, double (&out)[NV]) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (NV <= 32) {
-    int e;
-    const double s =
-        wave_sum_spread<(NV <= 32 ? NV : 1)>(*(const double(*)[NV <= 32 ? NV : 1])v, &e);
-    if (e >= 0) red[wv * 64 + e] = s;
-  } else {
-    double s[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) s[i] = v[i];
-    wave_sum_n<NV>(s);
-    if (lane == 0)
-#pragma unroll
-      for (int i = 0; i < NV; ++i) red[wv * 64 + i] = s[i];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; ++i) out[i] = red[i] + red[64 + i] + red[128 + i] + red[192 + i];
-  __syncthreads();
-}
-
-template <int NV>
-__device__ __forceinline__ void block_max(double (&v)[NV], double* red, double (&out)[NV]) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = wave_max(v[i]);
-    if (lane == 0) red[wv * 64 + i] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; ++i)
-    out[i] = fmax(fmax(red[i], red[64 + i]), fmax(red[128 + i], red[192 + i]));
-  __syncthreads();
-}
-
-__device__ __forceinline__ double block_max1(double v, double* red) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  double s = wave_max(v);
-  if (lane == 0) red[wv * 64] = s;
-  __syncthreads();
-  double r = fmax(fmax(red[0], red[64]), fmax(red[128], red[192]));
-  __syncthreads();
-  return r;
-}
-
-// Reductions over the WPR waves of one restart (wave w of the restart at
-// red[w * 64 ...]): the sums take the waves in order, as block_sum does.
+// Reductions over the WPR waves of one restart (wave w of the restart writes
+// red[w * 64 ...]); every thread of the restart gets the result.  The sums
+// take the waves in order, after a fixed-tree sum within each wave.  Every
+// thread of the workgroup must call them (they hold workgroup barriers).
 template <int NV, int WPR>
 __device__ __forceinline__ void group_sum(double (&v)[NV], double* red, double (&out)[NV], int wv) {
   if (NV <= 32) {
