@@ -82,7 +82,8 @@ __global__ __launch_bounds__(256, 2) void gemm_h2_list_kernel(const H2Args* __re
 }
 
 // Workgroups [0, W): adjoint GEMM workers; [W, W + mc blocks): the MC stage,
-// one workgroup per (model, restart).
+// one workgroup per (model, restart), or per (model, two restarts) for a model
+// whose McArgs::nr is 2, as in its one-model step.
 template <int Q>
 __global__ __launch_bounds__(256, 2) void adjoint_mc_list_kernel(
     const H2Args* __restrict__ adj, const int4* __restrict__ tiles, int total, int W,
@@ -94,7 +95,10 @@ __global__ __launch_bounds__(256, 2) void adjoint_mc_list_kernel(
     const int blk = (int)blockIdx.x - W;
     const int m = find_item(mc_start, nm, blk);
     const McArgs a = mc[m];
-    mc_body<Q, float>(a, blk - mc_start[m], (double*)lds);
+    if (a.nr == 2)
+      mc_body<Q, float, 2>(a, blk - mc_start[m], (double*)lds);
+    else
+      mc_body<Q, float, 1>(a, blk - mc_start[m], (double*)lds);
   }
 }
 
@@ -271,7 +275,7 @@ int qlognei_batch_prepare(const bta_qlognei_batch_item* items, int nm, void* ws,
       return BTA_ERR_UNSUPPORTED;
     }
     cov_start[i + 1] = cov_start[i] + (int)cb;
-    mc_start[i + 1] = mc_start[i] + (int)A[i].b;
+    mc_start[i + 1] = mc_start[i] + (int)(A[i].mc.nr == 2 ? (A[i].b + 1) / 2 : A[i].b);
     fin_start[i + 1] = fin_start[i] + (int)fb;
     add_tiles(fwd[i], i, T.fwd);
     if (grad) add_tiles(adj[i], i, T.adj);

@@ -70,24 +70,6 @@ static bool fused_disabled() {
   return e && e[0] == '1';
 }
 
-// Two restarts per MC workgroup (two waves each) in the split-FP16 mixed mode
-// with q <= 5, whose MC workgroups share the CUs with the adjoint GEMM's
-// workers in one launch (one round of paired workgroups in the slots the
-// workers leave free is shorter than two rounds of single ones), when both
-// layouts fit MC_LDS_MAX and the cached root is in use (not the joint mode).
-// Every launch of that mode's MC stage (fused or alone, value-only or with the
-// gradient) takes the same choice, and a restart's results depend only on it,
-// not on the partner restart, so its value-only and value+grad steps, chunked
-// or whole, keep giving identical values.  Run alone over the whole GPU
-// (mixed_f32), single workgroups fill both slots of every CU and stay.
-// BTA_MC_PAIR=0 in the environment keeps one restart per workgroup (how the
-// two are compared in the tests).
-static bool mc_paired(const McArgs& a, int q, bool f32, bool split) {
-  const char* e = getenv("BTA_MC_PAIR");
-  const McLayout lay(q, a.n_b, a.S);
-  return f32 && split && q <= 5 && !a.Lb_levels && !(e && e[0] == '0') &&
-         (size_t)2 * lay.total * sizeof(double) <= MC_LDS_MAX;
-}
 
 // Launches the fused kernel when it applies (q <= 5, mixed sample arithmetic,
 // the GEMM planned with 64-column tiles, the MC layout within MC_LDS_MAX);
@@ -103,7 +85,7 @@ static int launch_adjoint_mc(const H2Args& args, const McArgs& a, int q, int64_t
   if (!f32 || q > 5 || bn != 64 || g.seg_chunks > 0 || fused_disabled() ||
       (size_t)lay.total * sizeof(double) > MC_LDS_MAX || b < 1 || b > (1 << 24))
     return -1;
-  if (mc_paired(a, q, f32, true)) {
+  if (a.nr == 2) {
     const dim3 grid((unsigned)(W + (b + 1) / 2));
     switch (q) {
       case 1: hipLaunchKernelGGL((adjoint_mc_kernel<1, float, 2>), grid, dim3(256), 0, s, g, W, a); break;
@@ -135,7 +117,7 @@ static int launch_mc_q(const McArgs& a, int64_t b, bool f32, bool split, hipStre
               a.S);
     return BTA_ERR_UNSUPPORTED;
   }
-  if (mc_paired(a, Q, f32, split))
+  if (a.nr == 2)
     hipLaunchKernelGGL((qlognei_mc_kernel<Q, float, 2>), dim3((unsigned)((b + 1) / 2)), dim3(256),
                        2 * bytes, s, a);
   else if (f32)
@@ -343,6 +325,7 @@ int qlognei_step_args(const bta_gp_model* m, const bta_qlognei_state* st, const 
   a.tau_relu = st->tau_relu;
   a.tau_max = st->tau_max;
   a.jitter = st->jitter;
+  a.nr = mc_paired(a, st->q, f32, split) ? 2 : 1;
   // dK_x = gq U per restart block, contracted with dK/dxs on the fly
   QsGradArgs& g = A.g;
   g.xs = xs;

@@ -3,6 +3,8 @@
 // one-model step (qlognei.hip) and the step over many models (qbatch.hip).
 // The algorithm is described at the top of qlognei.hip.
 #pragma once
+#include <cstdlib>
+
 #include "bta_common.h"
 #include "gemm.h"
 #include "gemm_h2_body.h"
@@ -37,6 +39,7 @@ struct McArgs {
   int* status;
   int d, n_pad, n_b, S, kind, max_tries, grad;
   int nrest;  // restarts (b)
+  int nr;     // restarts per MC workgroup (1 or 2; mc_paired)
   double outputscale, y_mean, y_std, c, tau_relu, tau_max, jitter;
 };
 
@@ -71,6 +74,26 @@ struct McLayout {
     total = o;
   }
 };
+
+// Two restarts per MC workgroup (two waves each) in the split-FP16 mixed mode
+// with q <= 5, whose MC workgroups share the CUs with the adjoint GEMM's
+// workers in one launch (one round of paired workgroups in the slots the
+// workers leave free is shorter than two rounds of single ones), when both
+// layouts fit MC_LDS_MAX and the cached root is in use (not the joint mode).
+// Every launch of that mode's MC stage (fused or alone, value-only or with the
+// gradient, one model or many) takes the same choice, kept in McArgs::nr when
+// the step's arguments are built, and a restart's results depend only on it,
+// not on the partner restart, so value-only and value+grad steps, chunked or
+// whole, one model or many, keep giving identical values.  Run alone over the
+// whole GPU (mixed_f32), single workgroups fill both slots of every CU and
+// stay.  BTA_MC_PAIR=0 in the environment keeps one restart per workgroup (how
+// the two are compared in the tests).
+inline bool mc_paired(const McArgs& a, int q, bool f32, bool split) {
+  const char* e = getenv("BTA_MC_PAIR");
+  const McLayout lay(q, a.n_b, a.S);
+  return f32 && split && q <= 5 && !a.Lb_levels && !(e && e[0] == '0') &&
+         (size_t)2 * lay.total * sizeof(double) <= MC_LDS_MAX;
+}
 
 // a / b in TM: FP32 takes the hardware reciprocal (1 ulp) and a product, FP64
 // divides exactly

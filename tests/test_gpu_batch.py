@@ -56,24 +56,27 @@ def _models():
 
 
 def test_batched_step_equals_one_model_steps():
-    """The batched launches run one restart per MC workgroup, as the one-model
-    step does with BTA_MC_PAIR=0: the results agree bit for bit (the paired
-    one-model launch is checked against the single one in test_gpu_mixed.py)."""
+    """Every model's restarts run the one-model step's bodies with the same MC
+    workgroup choice (two restarts per workgroup by default, one with
+    BTA_MC_PAIR=0, fixed when the batch is prepared): values, gradients and
+    status words equal the one-model steps' bit for bit in both modes."""
     import os
     ws, acqs, _ = _models()
     ns = [w["X"].shape[0] for w in ws]
-    _say(f"batched step: {len(ws)} models, n {min(ns)}..{max(ns)}, b=256 q=2 S=256 mixed")
-    batch = B.BatchedQLogNEI(acqs, 256, q=2)
-    vals, grads = batch.value_and_grad([w["Xq"].to(DEV) for w in ws])
     old = os.environ.get("BTA_MC_PAIR")
-    os.environ["BTA_MC_PAIR"] = "0"
     try:
-        for i, (w, acq) in enumerate(zip(ws, acqs)):
-            v1, g1, st1 = acq._launch(w["Xq"].to(DEV).contiguous(), True)
-            torch.cuda.synchronize()
-            assert torch.equal(batch.status[i], st1), i
-            assert torch.equal(vals[i], v1), i
-            assert torch.equal(grads[i], g1), i
+        for pair in ("1", "0"):
+            os.environ["BTA_MC_PAIR"] = pair
+            _say(f"batched step: {len(ws)} models, n {min(ns)}..{max(ns)}, b=256 q=2 S=256 "
+                 f"mixed, BTA_MC_PAIR={pair}")
+            batch = B.BatchedQLogNEI(acqs, 256, q=2)
+            vals, grads = batch.value_and_grad([w["Xq"].to(DEV) for w in ws])
+            for i, (w, acq) in enumerate(zip(ws, acqs)):
+                v1, g1, st1 = acq._launch(w["Xq"].to(DEV).contiguous(), True)
+                torch.cuda.synchronize()
+                assert torch.equal(batch.status[i], st1), (pair, i)
+                assert torch.equal(vals[i], v1), (pair, i)
+                assert torch.equal(grads[i], g1), (pair, i)
     finally:
         if old is None:
             del os.environ["BTA_MC_PAIR"]
