@@ -197,15 +197,16 @@ def test_fused_adjoint_equals_sequential_path():
         assert torch.equal(v_f, v_s) and torch.equal(g_f, g_s) and torch.equal(s_f, s_s), pair
 
 
-@pytest.mark.parametrize("q,b", [(4, 127), (2, 128), (5, 33), (1, 1)])
-def test_fused_paired_restarts_match_single(q, b):
+@pytest.mark.parametrize("q,b,d", [(4, 127, 6), (2, 128, 6), (5, 33, 6), (1, 1, 6), (5, 31, 30)])
+def test_fused_paired_restarts_match_single(q, b, d):
     """The fused launch runs two restarts per MC workgroup (two waves each): each
     restart's reductions then run over two waves instead of four, so values and
     gradients differ from the one-restart launch by rounding only, and the
     status words agree.  An odd restart count leaves the second half of the last
-    workgroup without a restart of its own."""
-    _say(f"paired vs single MC restarts n=1024 d=6 q={q} S=512 b={b} mixed")
-    acq, ref, Xq = _acq_pair(1024, 6, q, 512, "mixed", seed=5)
+    workgroup without a restart of its own, and q d > 128 (q = 5, d = 30) more
+    query coordinates than a restart's 128 threads."""
+    _say(f"paired vs single MC restarts n=1024 d={d} q={q} S=512 b={b} mixed")
+    acq, ref, Xq = _acq_pair(1024, d, q, 512, "mixed", seed=5)
     Xd = Xq[:b].to(DEV).contiguous()
     v_p, g_p, s_p = _launch_env(acq, Xd, {"BTA_MC_PAIR": "1"})
     v_1, g_1, s_1 = _launch_env(acq, Xd, {"BTA_MC_PAIR": "0"})
