@@ -242,9 +242,10 @@ __device__ __forceinline__ void mc_body(const McArgs& a, const int bw, double* s
   const TV* vt = (const TV*)a.vt;
   const int64_t r0 = (int64_t)b * Q;
 
-  // the query coordinates (one per thread), then the restart's V rows for
-  // V_b V_b^T (16-byte loads): their first two column passes are issued here,
-  // ahead of the other loads of the stage
+  // the query coordinates (the first TPR one per thread, the rest, for q d >
+  // TPR, after), then the restart's V rows for V_b V_b^T (16-byte loads): their
+  // first two column passes are issued here, ahead of the other loads of the
+  // stage
   const bool has_x = tid < Q * d;
   const double xr = has_x ? a.xs[r0 * d + tid] : 0.0;
   constexpr int VW = 16 / (int)sizeof(TV);  // elements per 16-byte load
